@@ -1,0 +1,209 @@
+/*
+ * scp3g.h - C ABI of the MI355X Hamiltonian-assembly library (libscp3g.so).
+ *
+ * The library replaces the hot loop of SCP3's -G (no-gradient) estimator,
+ * SCP3.cpp:187-232:
+ *
+ *   for each block of points:
+ *     y  = sobol::std_normal(...) / sqrt(2)           SCP3.cpp:196-201
+ *     r  = bohr * (x0 + MUa * y)                       SCP3.cpp:208
+ *     V  = pot(nw, r) / autokcalpmol                   SCP3.cpp:219
+ *     bV = V - 0.5 * y . (omega o y)                   SCP3.cpp:222-223
+ *     M += bra diag(bV) bra^T   (upper triangle)       SCP3.cpp:231,
+ *                                                      ScaledMatrixElements.cpp:951-971
+ *   M / P + HO diagonal                                SCP3.cpp:237-239,269-271
+ *
+ * and the eigensolver's lowest-K levels (eigensolver.cpp:20-109).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative code on failure;
+ *    scp3g_last_error() returns the message.  Nothing in the library
+ *    exits or throws across this boundary (the reference exits inside
+ *    ScaledMatrixElements.cpp:119-122 and h2o.h:42-45).
+ *  - Matrices follow the reference: column-major, M(r, c) at c*n + r, only
+ *    the upper triangle (r <= c) is accumulated.
+ *  - Pointers named *_host are host memory borrowed for the call; pointers
+ *    named *_dev are device memory on the context's device, and the call
+ *    is ordered on the context's stream (scp3g_sync waits for it).
+ *  - One context per host thread; a context is not re-entrant (as an
+ *    h2o::PES instance is not, qtip4pf.cpp:137-148).
+ */
+#ifndef SCP3G_H
+#define SCP3G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct scp3g_ctx scp3g_ctx;
+
+/* precision modes of the H accumulation */
+#define SCP3G_FP64 0   /* FP64 potential, FP64 basis, FP64 MFMA SYRK          */
+#define SCP3G_MIXED 1  /* FP32 potential, FP32 basis, 3-term split-BF16 MFMA  */
+                       /* SYRK accumulated in FP32 per tile, folded into an   */
+                       /* FP64 running H by compensated summation             */
+
+/* potentials selectable on the GPU (h2o.h:20-48 names, plus ttm3f) */
+#define SCP3G_PES_QTIP4PF 0
+#define SCP3G_PES_TTM3F 1
+
+/* ---- context ------------------------------------------------------- */
+
+/* Create a context on HIP device `device` (its own non-blocking stream). */
+int scp3g_create(scp3g_ctx **ctx, int device);
+void scp3g_destroy(scp3g_ctx *ctx);
+const char *scp3g_last_error(const scp3g_ctx *ctx);
+int scp3g_sync(scp3g_ctx *ctx);
+
+/*
+ * Molecular system: nw waters, x0_host[9nw] equilibrium Cartesians in bohr
+ * in O H H order (after OHHOHH, DiskIO.cpp:142-195), omega_host[nmodes0]
+ * harmonic frequencies in hartree, MUa_host[9nw x nmodes0] column-major
+ * mass-and-frequency-scaled normal modes (SCP3.cpp:307-314).
+ * Replaces the arguments of SCP3_a (SCP3.cpp:154).
+ */
+int scp3g_set_system(scp3g_ctx *ctx, int nw, const double *x0_host, int nmodes0,
+                     const double *omega_host, const double *MUa_host);
+
+/*
+ * Basis: selected modes (indices into 0..nmodes0-1, ascending frequency),
+ * singles over all selected modes, doubles over the first nmodes2 and
+ * triples over the first nmodes3 of them.  Replaces the constructor of
+ * ScaledMatrixElements (ScaledMatrixElements.h:73-83).
+ */
+int scp3g_set_basis(scp3g_ctx *ctx, int nsel, const int *modes_host, int nmodes2,
+                    int nmodes3);
+/* n = 1 + nsel + nmodes2(nmodes2+1)/2 + nmodes3(nmodes3+1)(nmodes3+2)/6,
+ * ScaledMatrixElements.cpp:73-78 */
+int scp3g_basis_size(const scp3g_ctx *ctx, int64_t *n);
+/* ScaledMatrixElements.cpp:81-99 */
+int scp3g_sub_basis_size(const scp3g_ctx *ctx, int level, int64_t *n);
+
+/* Potential by name: "qtip4pf" or "ttm3f" (PESFromString, h2o.h:20-48). */
+int scp3g_set_pes(scp3g_ctx *ctx, const char *name);
+/* SCP3G_FP64 or SCP3G_MIXED */
+int scp3g_set_precision(scp3g_ctx *ctx, int mode);
+/* points per device tile (multiple of 256; default 4096) */
+int scp3g_set_tile(scp3g_ctx *ctx, int tile);
+
+/* ---- whole-path entry points -------------------------------------- */
+
+/* Zero the running sum M and the point counter. */
+int scp3g_reset(scp3g_ctx *ctx);
+
+/*
+ * Accumulate Sobol points with seeds skip .. skip+npts-1 (the reference's
+ * sobol_skip, SCP3.cpp:48,196-199) into the running FP64 sum M.
+ */
+int scp3g_accumulate(scp3g_ctx *ctx, int64_t skip, int64_t npts);
+
+/*
+ * Accumulate caller-supplied standard normals (the -r rng-file branch,
+ * SCP3.cpp:188-194): normals_host[npts x nmodes0] point-major; they are
+ * divided by sqrt(2) as SCP3.cpp:201 does.
+ */
+int scp3g_accumulate_normals(scp3g_ctx *ctx, const double *normals_host, int64_t npts);
+
+/* Number of points accumulated since the last reset. */
+int scp3g_points_done(const scp3g_ctx *ctx, int64_t *npts);
+
+/*
+ * Mout = M / points_done + HO diagonal + 0.5*(sum(omega) - sum(omega_sel))
+ * (SCP3.cpp:237-239,269-271; ScaledMatrixElements.cpp:735-784), written to
+ * M_host[n*n] column-major with the upper triangle filled and the strict
+ * lower triangle zero - the layout of the reference's M and of the sM_*.h5
+ * dataset.  raw != 0 returns the plain running sum M instead.
+ */
+int scp3g_h_download(scp3g_ctx *ctx, double *M_host, int raw);
+
+/*
+ * Lowest k eigenvalues (hartree, ascending) of Mout, computed on the GPU in
+ * FP64 (replaces eigensolver.cpp:20-80 dsyevr RANGE='I').  evecs_host may be
+ * NULL; otherwise n x k column-major.  tol is the residual tolerance in
+ * hartree (<= 0 selects 1e-9).
+ */
+int scp3g_lowest_levels(scp3g_ctx *ctx, int k, double tol, double *evals_host,
+                        double *evecs_host);
+
+/* ---- per-stage entry points (device pointers; used by parity tests) ---- */
+
+/*
+ * y_dev[d*ldy + i] = BSM(sobol(skip+i)[d]) / (halve ? sqrt(2) : 1) for
+ * d < dims, i < npts (sobol.cpp:14198-27867, beasley_springer_moro.cpp:14-46,
+ * SCP3.cpp:201).  ints_dev (nullable) receives the 62-bit Sobol integers
+ * (lastq of i8_sobol) in the same layout.
+ */
+int scp3g_sobol_normals(scp3g_ctx *ctx, int dims, int64_t skip, int64_t npts, int halve,
+                        double *y_dev, int64_t ldy, uint64_t *ints_dev);
+
+/*
+ * crd_dev[c*ldc + i] = bohr*(x0[c] + sum_k MUa[c,k] y[k*ldy + i]) for
+ * c < 9nw (SCP3.cpp:208), in Angstrom; fp32 != 0 writes float.
+ */
+int scp3g_map_coords(scp3g_ctx *ctx, const double *y_dev, int64_t ldy, int64_t npts,
+                     void *crd_dev, int64_t ldc, int fp32);
+
+/*
+ * Energies of npts configurations crd_dev[c*ldc + i] (Angstrom, O H H
+ * order) in kcal/mol: E_dev[i] = pot(nw, crd_i) (PES.h:13-23 energy-only
+ * operator()).  fp32 != 0 reads float coordinates and evaluates in FP32.
+ */
+int scp3g_pes_energy(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const void *crd_dev,
+                     int64_t ldc, int fp32, double *E_dev);
+
+/*
+ * Basis table B_dev[a*ldb + i] = bra_a(q_i) for the context's basis, with
+ * q_i[m] = y_dev[modes[m]*ldy + i] (get_bra, ScaledMatrixElements.cpp:870-930).
+ * fp32 != 0 writes float.
+ */
+int scp3g_basis_table(scp3g_ctx *ctx, const double *y_dev, int64_t ldy, int64_t npts,
+                      void *B_dev, int64_t ldb, int fp32);
+
+/*
+ * H_dev (n x n column-major, upper) += B diag(w) B^T over npts points, for
+ * B_dev[a*ldb + i] (n rows).  mode SCP3G_FP64 reads double B; SCP3G_MIXED
+ * reads float B and folds an FP32 tile sum into H_dev with the
+ * compensation array Hc_dev (same shape, may not be NULL in that mode).
+ * Replaces addEpot's dsyr2k (ScaledMatrixElements.cpp:951-971).
+ */
+int scp3g_h_accumulate(scp3g_ctx *ctx, int mode, int64_t n, const void *B_dev, int64_t ldb,
+                       const double *w_dev, int64_t npts, double *H_dev, double *Hc_dev);
+
+/* ---- host-side set-up (SCP3.cpp:292-318) --------------------------- */
+
+/*
+ * Normal modes from a mass-weighted Hessian (3N x 3N, atomic units):
+ * eigen-decomposition (ascending), omega = sqrt(ev[6:]),
+ * MUa[:, k] = U[:, 6+k] / (sqrt(mass) * sqrt(omega_k)).  omega_all_host
+ * (nullable) receives all 3N sqrt(|ev|) (omega0.dat, SCP3.cpp:300-302).
+ */
+int scp3g_normal_modes(int n3, const double *mass_host, const double *hess_host,
+                       double *omega_host, double *MUa_host, double *omega_all_host);
+
+/*
+ * CPU FP64 energy of one configuration by potential name ("qtip4pf",
+ * "ttm3f"), kcal/mol, with a central-difference gradient when grad_host is
+ * not NULL (PES.h interface; used for input preparation).
+ */
+int scp3g_pes_cpu(const char *name, int nw, const double *crd_host, double *E_host,
+                  double *grad_host);
+
+/* ---- timing ------------------------------------------------------- */
+
+/* Enable/disable per-kernel HIP-event timing on the context's stream. */
+int scp3g_set_timing(scp3g_ctx *ctx, int on);
+/*
+ * Total milliseconds and launch count of kernel `name` since timing was
+ * enabled ("sobol", "map", "pes", "basis", "syrk", "fold", "finalize",
+ * "symv").  Synchronises the stream.
+ */
+int scp3g_kernel_time(scp3g_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCP3G_H */

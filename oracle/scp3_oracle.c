@@ -1,0 +1,454 @@
+/*
+ * CPU FP64 oracle for SCP3's Hamiltonian-assembly path (the -G, no-gradient
+ * estimator).  TEST INFRASTRUCTURE ONLY: this file is the checker that the
+ * HIP path is compared against.  It is loaded by tests/, by the smoke check
+ * and by bench.py's cpu_baseline leg, never by the product library.
+ *
+ * Each function restates the reference algorithm in plain C and cites the
+ * reference file:line it follows.  Floating-point operation order follows
+ * the reference where the reference fixes it (BSM, Hermite functions,
+ * get_bra, the potential); BLAS dsyr2k's internal summation order is not
+ * fixed by the reference and is restated as a plain point-ordered sum.
+ *
+ * Parity pins (tests/test_oracle.py):
+ *   sobol        - reference build oracle/_ref (i8_sobol), Gray-code closed
+ *                  form, van der Corput on dimension 0, committed fixtures
+ *   bsm          - reference build, symmetry, scipy ndtri within 3e-9
+ *   basis/bra    - Python brute force over the reference index order,
+ *                  Gauss-Hermite orthonormality of the product basis
+ *   accumulate   - numpy B^T diag(V) B; harmonic-potential identity
+ *   qtip4pf      - reference build energies, monomer minimum, FD gradient
+ *   ttm3f        - reference build energies, direct linear solve of the
+ *                  induced dipoles, gammq23 vs scipy
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../scp3_amd/csrc/sobol_dirnums.inc"
+
+/* Constants.cpp:12-18, Constants.h:21 */
+static const double ORC_BOHR = 0.52917721092;
+static const double ORC_AUTOKCALPMOL = 627.509474277194;
+
+/* ------------------------------------------------------------------ */
+/* Sobol (sobol.cpp:14198-27867)                                        */
+/* ------------------------------------------------------------------ */
+
+#define ORC_MAXCOL 62
+
+/* Direction integers v[d][j] scaled by 2^(61-j), sobol.cpp:27700-27780. */
+int orc_sobol_table(int dims, uint64_t *v)
+{
+    if (dims < 1 || dims > SCP3_SOBOL_DIM_MAX) return -1;
+    for (int d = 0; d < dims; d++) {
+        uint64_t *row = v + (size_t)d * ORC_MAXCOL;
+        if (d == 0) {                          /* sobol.cpp:27706-27709 */
+            for (int j = 0; j < ORC_MAXCOL; j++) row[j] = 1;
+        } else {
+            unsigned p = scp3_sobol_poly[d];
+            int m = 0;
+            for (unsigned t = p; t > 1; t >>= 1) m++;   /* :27720-27731 */
+            int includ[16];
+            unsigned t = p;
+            for (int k = m - 1; k >= 0; k--) {          /* :27736-27742 */
+                includ[k] = (int)(t & 1u);
+                t >>= 1;
+            }
+            for (int j = 0; j < m; j++) row[j] = scp3_sobol_init[d][j];
+            for (int j = m; j < ORC_MAXCOL; j++) {      /* :27749-27765 */
+                uint64_t newv = row[j - m];
+                uint64_t l = 1;
+                for (int k = 0; k < m; k++) {
+                    l <<= 1;
+                    if (includ[k]) newv ^= l * row[j - k - 1];
+                }
+                row[j] = newv;
+            }
+        }
+    }
+    /* :27769-27780  column j scaled by 2^(maxcol-1-j) */
+    for (int d = 0; d < dims; d++)
+        for (int j = 0; j < ORC_MAXCOL; j++)
+            v[(size_t)d * ORC_MAXCOL + j] <<= (ORC_MAXCOL - 1 - j);
+    return 0;
+}
+
+/* position (1-based) of the lowest zero bit, sobol.cpp:14121-14196 */
+static int orc_bit_lo0(int64_t n)
+{
+    int bit = 0;
+    for (;;) {
+        bit++;
+        int64_t n2 = n / 2;
+        if (n == 2 * n2) break;
+        n = n2;
+    }
+    return bit;
+}
+
+/*
+ * Integer Sobol points for seeds seed0 .. seed0+npts-1, point-major
+ * out[i*dims + d].  Follows the Antonov-Saleev update of i8_sobol
+ * (sobol.cpp:27794-27858): the state before seed s is the XOR of
+ * v[lo0(t)-1] over t < s.  That XOR telescopes to the XOR of v[j] over the
+ * set bits j of the Gray code s ^ (s >> 1) (lo0(t)-1 is the bit in which
+ * gray(t) and gray(t+1) differ), which replaces the reference's 2^30-step
+ * catch-up loop (sobol.cpp:27823-27836) for the starting state; from there
+ * the points follow the reference's single-step update.
+ */
+int orc_sobol_ints(int dims, int64_t seed0, int64_t npts, uint64_t *out)
+{
+    uint64_t *v = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)dims * ORC_MAXCOL);
+    uint64_t *lastq = (uint64_t *)calloc((size_t)dims, sizeof(uint64_t));
+    if (!v || !lastq) { free(v); free(lastq); return -2; }
+    if (orc_sobol_table(dims, v)) { free(v); free(lastq); return -1; }
+    if (seed0 < 0) seed0 = 0;
+    {
+        const uint64_t g = (uint64_t)seed0 ^ ((uint64_t)seed0 >> 1);
+        for (int j = 0; j < ORC_MAXCOL; j++)
+            if ((g >> j) & 1u)
+                for (int d = 0; d < dims; d++) lastq[d] ^= v[(size_t)d * ORC_MAXCOL + j];
+    }
+    for (int64_t i = 0; i < npts; i++) {
+        int64_t s = seed0 + i;
+        int l = orc_bit_lo0(s);
+        if (l > ORC_MAXCOL) { free(v); free(lastq); return -3; }
+        for (int d = 0; d < dims; d++) {
+            out[(size_t)i * dims + d] = lastq[d];
+            lastq[d] ^= v[(size_t)d * ORC_MAXCOL + l - 1];
+        }
+    }
+    free(v);
+    free(lastq);
+    return 0;
+}
+
+/* Beasley-Springer-Moro inverse normal, beasley_springer_moro.cpp:14-46 */
+void orc_bsm(int64_t n, const double *u, double *x)
+{
+    static const double a[] = {2.50662823884, -18.61500062529, 41.39119773534, -25.44106049637};
+    static const double b[] = {-8.47351093090, 23.08336743743, -21.06224101826, 3.13082909833};
+    static const double c[] = {0.3374754822726147, 0.9761690190917186, 0.1607979714918209,
+        0.0276438810333863, 0.0038405729373609, 0.0003951896511919,
+        0.0000321767881768, 0.0000002888167364, 0.0000003960315187};
+    for (int64_t j = 0; j < n; j++) {
+        double yj = u[j] - 0.5;
+        double r = 0.0;
+        if (fabs(yj) < 0.42) {
+            r = yj * yj;
+            x[j] = yj * (((a[3] * r + a[2]) * r + a[1]) * r + a[0])
+                   / ((((b[3] * r + b[2]) * r + b[1]) * r + b[0]) * r + 1);
+        } else {
+            if (yj > 0) r = log(-log(1 - u[j]));
+            else if (yj < 0) r = log(-log(u[j]));
+            x[j] = c[0] + r * (c[1] + r * (c[2] + r * (c[3] + r * (c[4] + r * (c[5]
+                   + r * (c[6] + r * (c[7] + r * c[8])))))));
+            if (yj < 0) x[j] = -x[j];
+        }
+    }
+}
+
+/*
+ * Standard normals of SCP3's sampler: sobol::std_normal (sobol.cpp:28706)
+ * = i8_sobol (quasi = lastq * 2^-62, :27846) then BSM; SCP3.cpp:201
+ * divides by sqrt(2).  Output point-major y[i*dims + d] (the column of
+ * `by` for point i), already divided by sqrt(2) when halve != 0.
+ */
+int orc_sobol_normals(int dims, int64_t seed0, int64_t npts, int halve, double *y)
+{
+    uint64_t *q = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)dims * (size_t)npts);
+    if (!q) return -2;
+    int rc = orc_sobol_ints(dims, seed0, npts, q);
+    if (rc) { free(q); return rc; }
+    const double recipd = 1.0 / (double)((uint64_t)1 << 62);     /* :27780 */
+    size_t tot = (size_t)dims * (size_t)npts;
+    for (size_t k = 0; k < tot; k++) y[k] = (double)q[k] * recipd;
+    orc_bsm((int64_t)tot, y, y);
+    if (halve) {
+        const double s2 = sqrt(2.0);
+        for (size_t k = 0; k < tot; k++) y[k] /= s2;
+    }
+    free(q);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Basis (HarmonicOscillatorBasis.cpp:16-46, ScaledMatrixElements.*)    */
+/* ------------------------------------------------------------------ */
+
+static double orc_ho1(double y) { return sqrt(2.0) * y; }
+static double orc_ho2(double y) { return (2.0 * y * y - 1.0) / sqrt(2.0); }
+static double orc_ho3(double y) { return (2.0 * y * y - 3.0) * y / sqrt(3.0); }
+
+double orc_ho(int n, double y)
+{
+    switch (n) {
+    case 0: return 1.0;
+    case 1: return orc_ho1(y);
+    case 2: return orc_ho2(y);
+    case 3: return orc_ho3(y);
+    default: return NAN;
+    }
+}
+
+/* ScaledMatrixElements.cpp:73-78 */
+int64_t orc_basis_size(int nmodes, int nmodes2, int nmodes3)
+{
+    int64_t n2 = 1 + (int64_t)nmodes + (int64_t)nmodes2 * (nmodes2 + 1) / 2;
+    return n2 + ((int64_t)nmodes3 * (1 + nmodes3) * (2 + nmodes3)) / 6;
+}
+
+/* ScaledMatrixElements.cpp:81-99 */
+int64_t orc_sub_basis_size(int nmodes, int nmodes2, int nmodes3, int level)
+{
+    int64_t n = 0;
+    if (level >= 3) n += ((int64_t)nmodes3 * (1 + nmodes3) * (2 + nmodes3)) / 6;
+    if (level >= 2) n += (int64_t)nmodes2 * (nmodes2 + 1) / 2;
+    if (level >= 1) n += nmodes;
+    if (level >= 0) n += 1;
+    return n;
+}
+
+/* get_bra, ScaledMatrixElements.cpp:870-930: q holds the selected modes */
+int64_t orc_get_bra(int nmodes, int nmodes2, int nmodes3, const double *q, double *bra)
+{
+    int64_t R = 0;
+    bra[R++] = 1.0;
+    for (int m = 0; m < nmodes; m++) bra[R++] = orc_ho1(q[m]);
+    for (int m = 0; m < nmodes2; m++) bra[R++] = orc_ho2(q[m]);
+    for (int m = 0; m < nmodes2; m++) {
+        double bm = orc_ho1(q[m]);
+        for (int n = m + 1; n < nmodes2; n++) bra[R++] = bm * orc_ho1(q[n]);
+    }
+    for (int m = 0; m < nmodes3; m++) bra[R++] = orc_ho3(q[m]);
+    for (int m = 0; m < nmodes3; m++) {
+        double bm = orc_ho1(q[m]);
+        double bm2 = orc_ho2(q[m]);
+        for (int n = m + 1; n < nmodes3; n++) {
+            bra[R++] = bm2 * orc_ho1(q[n]);
+            bra[R++] = bm * orc_ho2(q[n]);
+        }
+    }
+    for (int m = 0; m < nmodes3; m++) {
+        double bm = orc_ho1(q[m]);
+        for (int n = m + 1; n < nmodes3; n++) {
+            double bmn = bm * orc_ho1(q[n]);
+            for (int p = n + 1; p < nmodes3; p++) bra[R++] = bmn * orc_ho1(q[p]);
+        }
+    }
+    return R;
+}
+
+/*
+ * addEpot(q, V, M), ScaledMatrixElements.cpp:951-971: ket = bra * V, then
+ * dsyr2k('U','N',0.5,bra,ket,1,M): upper triangle (r <= c) of the
+ * column-major n x n matrix M gains 0.5*sum_p (bra_r ket_c + ket_r bra_c).
+ * bra is n x P column-major (bra[p*n + r]).
+ */
+void orc_add_epot(int64_t n, int64_t P, const double *bra, const double *V, double *M)
+{
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t c = 0; c < n; c++) {
+        for (int64_t r = 0; r <= c; r++) {
+            double s = 0.0;
+            for (int64_t p = 0; p < P; p++) {
+                const double *b = bra + p * n;
+                s += b[r] * (b[c] * V[p]) + (b[r] * V[p]) * b[c];
+            }
+            M[c * n + r] += 0.5 * s;
+        }
+    }
+}
+
+/* addHODiagonal, ScaledMatrixElements.cpp:735-784 (omega of the selected modes) */
+void orc_add_ho_diagonal(int nmodes, int nmodes2, int nmodes3, const double *omega,
+                         int64_t n, double *M)
+{
+    double E0 = 0.0;
+    for (int k = 0; k < nmodes; k++) E0 += omega[k];
+    E0 *= 0.5;
+    int64_t C = 0;
+    M[C * n + C] += E0;
+    C = 1;
+    for (int k = 0; k < nmodes; k++, C++) M[C * n + C] += E0 + omega[k];
+    for (int k = 0; k < nmodes2; k++, C++) M[C * n + C] += E0 + 2 * omega[k];
+    for (int k = 0; k < nmodes2; k++)
+        for (int l = k + 1; l < nmodes2; l++, C++) M[C * n + C] += E0 + omega[k] + omega[l];
+    for (int k = 0; k < nmodes3; k++, C++) M[C * n + C] += E0 + 3.0 * omega[k];
+    for (int k = 0; k < nmodes3; k++)
+        for (int l = k + 1; l < nmodes3; l++) {
+            M[C * n + C] += E0 + 2.0 * omega[k] + omega[l];
+            C++;
+            M[C * n + C] += E0 + omega[k] + 2.0 * omega[l];
+            C++;
+        }
+    for (int k = 0; k < nmodes3; k++)
+        for (int l = k + 1; l < nmodes3; l++)
+            for (int j = l + 1; j < nmodes3; j++, C++)
+                M[C * n + C] += E0 + omega[k] + omega[l] + omega[j];
+}
+
+/* ------------------------------------------------------------------ */
+/* q-TIP4P/F energy, qtip4pf.cpp:19-248                                 */
+/* ------------------------------------------------------------------ */
+
+static const double Q_CHARGECON = 18.22234397655801030455;
+#define Q_qM (-1.1128 * Q_CHARGECON)
+#define Q_qH (-Q_qM / 2)
+static const double Q_rOHeq = 0.9419, Q_aHOHeq = 107.4;
+static const double Q_epsilon = 0.1852, Q_sigma = 3.1589;
+static const double Q_alpha_r = 2.287, Q_D_r = 116.09, Q_k_theta = 87.85;
+static const double Q_gammaM = 0.73612;
+#define Q_gamma1 ((1.0 - Q_gammaM) / 2)
+
+static double orc_qtip4pf_monomer(const double *w)      /* qtip4pf.cpp:57-107 */
+{
+    double d1[3], d2[3], r1 = 0, r2 = 0, d12 = 0;
+    for (int i = 0; i < 3; i++) {
+        d1[i] = w[i + 3] - w[i];
+        r1 += d1[i] * d1[i];
+        d2[i] = w[i + 6] - w[i];
+        r2 += d2[i] * d2[i];
+        d12 += d1[i] * d2[i];
+    }
+    r1 = sqrt(r1);
+    r2 = sqrt(r2);
+    const double x1 = Q_alpha_r * (r1 - Q_rOHeq);
+    const double x2 = Q_alpha_r * (r2 - Q_rOHeq);
+    const double V1 = x1 * x1 * (1.0 - x1 * (1.0 - (7.0 / 12.0) * x1)) * Q_D_r;
+    const double V2 = x2 * x2 * (1.0 - x2 * (1.0 - (7.0 / 12.0) * x2)) * Q_D_r;
+    const double r12 = 1.0 / r1 / r2;
+    const double xi = d12 * r12;
+    const double theta = acos(xi);
+    const double d_theta = theta - Q_aHOHeq * M_PI / 180.0;
+    return V1 + V2 + 0.5 * Q_k_theta * d_theta * d_theta;
+}
+
+/* qtip4pf::operator()(nw, crd), qtip4pf.cpp:152-248; crd in A, O H H order */
+double orc_qtip4pf(int64_t nw, const double *crd)
+{
+    double *msite = (double *)malloc(sizeof(double) * 3 * (size_t)(nw > 0 ? nw : 1));
+    double Eint = 0;
+    for (int64_t i = 0; i < nw; i++) {
+        const double *O = crd + 9 * i, *H1 = O + 3, *H2 = O + 6;
+        for (int k = 0; k < 3; k++) msite[3 * i + k] = Q_gammaM * O[k] + Q_gamma1 * (H1[k] + H2[k]);
+        Eint += orc_qtip4pf_monomer(crd + 9 * i);
+    }
+    double Evdw = 0, Eelec = 0;
+    for (int64_t i = 0; i < nw; i++) {
+        const int64_t i3 = 3 * i, i9 = 9 * i;
+        for (int64_t j = 0; j < nw; j++) {                 /* :179-198 */
+            if (i == j) continue;
+            for (int l = 1; l < 3; l++) {
+                const int64_t jh = 3 * (3 * j + l);
+                double Rsq = 0;
+                for (int k = 0; k < 3; k++) {
+                    const double dx = msite[i3 + k] - crd[jh + k];
+                    Rsq += dx * dx;
+                }
+                Eelec += Q_qH * Q_qM / sqrt(Rsq);
+            }
+        }
+        for (int64_t j = i + 1; j < nw; j++) {             /* :202-244 */
+            const int64_t j3 = 3 * j, j9 = 9 * j;
+            for (int a = 1; a < 3; a++)
+                for (int b = 1; b < 3; b++) {
+                    double Rsq = 0;
+                    for (int k = 0; k < 3; k++) {
+                        const double dx = crd[3 * (i3 + a) + k] - crd[3 * (j3 + b) + k];
+                        Rsq += dx * dx;
+                    }
+                    Eelec += Q_qH * Q_qH / sqrt(Rsq);
+                }
+            double Rsq = 0;
+            for (int k = 0; k < 3; k++) {
+                const double dx = crd[i9 + k] - crd[j9 + k];
+                Rsq += dx * dx;
+            }
+            Rsq = Q_sigma * Q_sigma / Rsq;
+            const double dR6 = Rsq * Rsq * Rsq;
+            Evdw += 4 * Q_epsilon * dR6 * (dR6 - 1.0);
+            Rsq = 0.0;
+            for (int k = 0; k < 3; k++) {
+                const double dx = msite[i3 + k] - msite[j3 + k];
+                Rsq += dx * dx;
+            }
+            Eelec += Q_qM * Q_qM / sqrt(Rsq);
+        }
+    }
+    free(msite);
+    return Eint + Evdw + Eelec;
+}
+
+/* ------------------------------------------------------------------ */
+/* The SCP3_a block loop (SCP3.cpp:187-232), -G form                    */
+/* ------------------------------------------------------------------ */
+
+typedef double (*orc_pes_fn)(int64_t nw, const double *crd);
+
+/*
+ * Samples points seed0..seed0+npts-1 (or takes `normals` point-major,
+ * already standard normal, as the -r branch does, SCP3.cpp:188-194),
+ * maps them to Cartesians r = bohr*(x0 + MUa*y) (SCP3.cpp:208), evaluates
+ * the potential (SCP3.cpp:219), forms bV = V - 0.5 y.(omega o y)
+ * (SCP3.cpp:222-223) and accumulates M += bra diag(bV) bra^T over the
+ * selected modes (SCP3.cpp:231).  Also returns y (point-major, halved),
+ * the potential in hartree and bV when the pointers are non-null.
+ * pes: 0 = qtip4pf, 1 = ttm3f (if linked).
+ */
+double orc_ttm3f(int64_t nw, const double *crd) __attribute__((weak));
+
+int orc_scp3_accumulate(int nw, int nmodes0, const double *x0, const double *omega,
+                        const double *MUa, int nsel, const int *modes, int nmodes2,
+                        int nmodes3, int pes, int64_t seed0, int64_t npts,
+                        const double *normals, double *M, double *y_out,
+                        double *V_out, double *bV_out)
+{
+    const int n3 = 9 * nw;
+    const int64_t n = orc_basis_size(nsel, nmodes2, nmodes3);
+    orc_pes_fn fn = pes == 1 ? orc_ttm3f : orc_qtip4pf;
+    if (!fn) return -4;
+    double *y = (double *)malloc(sizeof(double) * (size_t)nmodes0 * (size_t)npts);
+    double *bra = (double *)malloc(sizeof(double) * (size_t)n * (size_t)npts);
+    double *bV = (double *)malloc(sizeof(double) * (size_t)npts);
+    if (!y || !bra || !bV) { free(y); free(bra); free(bV); return -2; }
+    if (normals) {
+        const double s2 = sqrt(2.0);
+        for (size_t k = 0; k < (size_t)nmodes0 * (size_t)npts; k++) y[k] = normals[k] / s2;
+    } else {
+        int rc = orc_sobol_normals(nmodes0, seed0, npts, 1, y);
+        if (rc) { free(y); free(bra); free(bV); return rc; }
+    }
+#pragma omp parallel
+    {
+        double *r = (double *)malloc(sizeof(double) * n3);
+        double *q = (double *)malloc(sizeof(double) * (nsel > 0 ? nsel : 1));
+#pragma omp for schedule(static)
+        for (int64_t j = 0; j < npts; j++) {
+            const double *yj = y + j * nmodes0;
+            for (int c = 0; c < n3; c++) {
+                double s = 0.0;
+                for (int k = 0; k < nmodes0; k++) s += MUa[(size_t)k * n3 + c] * yj[k];
+                r[c] = ORC_BOHR * (x0[c] + s);
+            }
+            double V = fn(nw, r) / ORC_AUTOKCALPMOL;
+            double h = 0.0;
+            for (int k = 0; k < nmodes0; k++) h += yj[k] * (omega[k] * yj[k]);
+            bV[j] = V - 0.5 * h;
+            if (V_out) V_out[j] = V;
+            for (int m = 0; m < nsel; m++) q[m] = yj[modes[m]];
+            orc_get_bra(nsel, nmodes2, nmodes3, q, bra + (size_t)j * n);
+        }
+        free(r);
+        free(q);
+    }
+    if (M) orc_add_epot(n, npts, bra, bV, M);
+    if (y_out) memcpy(y_out, y, sizeof(double) * (size_t)nmodes0 * (size_t)npts);
+    if (bV_out) memcpy(bV_out, bV, sizeof(double) * (size_t)npts);
+    free(y);
+    free(bra);
+    free(bV);
+    return 0;
+}

@@ -1,0 +1,238 @@
+"""MI355X Hamiltonian-assembly path of SCP3: Python bindings over libscp3g.
+
+The product is the C ABI in include/scp3g.h (libscp3g.so, HIP kernels for
+gfx950).  This module loads it with ctypes and wraps the context in a small
+class; it has no compute path of its own, and raises if the library is
+missing or no GPU is visible.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libscp3g.so")
+
+FP64 = 0
+MIXED = 1
+PES_QTIP4PF = 0
+PES_TTM3F = 1
+
+# Constants.cpp:12-18, Constants.h:21
+BOHR = 0.52917721092
+AUTOCM = 2.194746313e5
+AUTOKCALPMOL = 627.509474277194
+HMASS = 1837.15137
+CMASS = 21891.6543
+OMASS = CMASS * 15.9994 / 12.0107
+
+_lib = None
+
+_SIGS = {
+    "scp3g_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
+    "scp3g_destroy": (None, [ctypes.c_void_p]),
+    "scp3g_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "scp3g_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "scp3g_set_system": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "scp3g_set_basis": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int]),
+    "scp3g_basis_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "scp3g_sub_basis_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
+    "scp3g_set_pes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "scp3g_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "scp3g_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "scp3g_reset": (ctypes.c_int, [ctypes.c_void_p]),
+    "scp3g_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
+    "scp3g_accumulate_normals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "scp3g_points_done": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "scp3g_h_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "scp3g_lowest_levels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                           ctypes.c_void_p]),
+    "scp3g_sobol_normals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+    "scp3g_map_coords": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]),
+    "scp3g_pes_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]),
+    "scp3g_basis_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]),
+    "scp3g_h_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p,
+                                          ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                          ctypes.c_void_p]),
+    "scp3g_normal_modes": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "scp3g_pes_cpu": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]),
+    "scp3g_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "scp3g_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_int64)]),
+}
+
+
+def exported_symbols():
+    """Names the C ABI exports (include/scp3g.h)."""
+    return sorted(_SIGS)
+
+
+def lib():
+    """Load libscp3g.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libscp3g.so is not built: run python -m scp3_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Scp3Error(RuntimeError):
+    pass
+
+
+class Context:
+    """One libscp3g context (device, stream, system, basis, running H)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.scp3g_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise Scp3Error("scp3g_create failed (%d): no usable GPU %d" % (rc, device))
+        self._h = h
+        self.L = L
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            self.L.scp3g_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, rc):
+        if rc != 0:
+            msg = self.L.scp3g_last_error(self._h)
+            raise Scp3Error("libscp3g error %d: %s" % (rc, msg.decode() if msg else ""))
+
+    # set-up -------------------------------------------------------------
+    def set_system(self, x0_bohr, omega, MUa):
+        x0 = np.ascontiguousarray(x0_bohr, dtype=np.float64)
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        MU = np.asfortranarray(MUa, dtype=np.float64)
+        nw = x0.size // 9
+        self.check(self.L.scp3g_set_system(self._h, nw, _ptr(x0), om.size, _ptr(om), _ptr(MU)))
+        self.nw, self.nmodes0 = nw, om.size
+
+    def set_basis(self, modes, nmodes2, nmodes3):
+        m = np.ascontiguousarray(modes, dtype=np.int32)
+        self.check(self.L.scp3g_set_basis(self._h, m.size, _ptr(m), nmodes2, nmodes3))
+
+    def basis_size(self):
+        n = ctypes.c_int64()
+        self.check(self.L.scp3g_basis_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def sub_basis_size(self, level):
+        n = ctypes.c_int64()
+        self.check(self.L.scp3g_sub_basis_size(self._h, level, ctypes.byref(n)))
+        return n.value
+
+    def set_pes(self, name):
+        self.check(self.L.scp3g_set_pes(self._h, name.encode()))
+
+    def set_precision(self, mode):
+        self.check(self.L.scp3g_set_precision(self._h, mode))
+
+    def set_tile(self, tile):
+        self.check(self.L.scp3g_set_tile(self._h, tile))
+
+    # whole path ---------------------------------------------------------
+    def reset(self):
+        self.check(self.L.scp3g_reset(self._h))
+
+    def accumulate(self, skip, npts):
+        self.check(self.L.scp3g_accumulate(self._h, int(skip), int(npts)))
+
+    def accumulate_normals(self, normals):
+        a = np.ascontiguousarray(normals, dtype=np.float64)
+        self.check(self.L.scp3g_accumulate_normals(self._h, _ptr(a), a.shape[0]))
+
+    def points_done(self):
+        n = ctypes.c_int64()
+        self.check(self.L.scp3g_points_done(self._h, ctypes.byref(n)))
+        return n.value
+
+    def h_download(self, raw=False):
+        n = self.basis_size()
+        M = np.zeros((n, n), dtype=np.float64, order="F")
+        self.check(self.L.scp3g_h_download(self._h, _ptr(M), 1 if raw else 0))
+        return M
+
+    def lowest_levels(self, k, tol=0.0, vectors=False):
+        n = self.basis_size()
+        ev = np.zeros(k)
+        V = np.zeros((n, k), order="F") if vectors else None
+        self.check(self.L.scp3g_lowest_levels(self._h, k, tol, _ptr(ev), _ptr(V)))
+        return (ev, V) if vectors else ev
+
+    def sync(self):
+        self.check(self.L.scp3g_sync(self._h))
+
+    # timing -------------------------------------------------------------
+    def set_timing(self, on):
+        self.check(self.L.scp3g_set_timing(self._h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        self.check(self.L.scp3g_kernel_time(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+
+def pes_cpu(name, crd, grad=False):
+    """FP64 CPU energy (kcal/mol) of one configuration (Angstrom, O H H order)."""
+    L = lib()
+    x = np.ascontiguousarray(crd, dtype=np.float64).ravel()
+    E = np.zeros(1)
+    g = np.zeros_like(x) if grad else None
+    rc = L.scp3g_pes_cpu(name.encode(), x.size // 9, _ptr(x), _ptr(E), _ptr(g))
+    if rc != 0:
+        raise Scp3Error("scp3g_pes_cpu(%s) failed (%d)" % (name, rc))
+    return (E[0], g) if grad else E[0]
+
+
+def normal_modes(mass, hess):
+    """omega (hartree), MUa (3N x M col-major), omega_all = sqrt|ev| (SCP3.cpp:292-318)."""
+    L = lib()
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    H = np.asfortranarray(hess, dtype=np.float64)
+    n3 = mass.size
+    om = np.zeros(n3 - 6)
+    MU = np.zeros((n3, n3 - 6), order="F")
+    oall = np.zeros(n3)
+    rc = L.scp3g_normal_modes(n3, _ptr(mass), _ptr(H), _ptr(om), _ptr(MU), _ptr(oall))
+    if rc != 0:
+        raise Scp3Error("scp3g_normal_modes failed (%d)" % rc)
+    return om, MU, oall

@@ -1,0 +1,122 @@
+// Internal declarations shared by the HIP translation units of libscp3g.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/scp3g.h"
+
+namespace scp3g {
+
+// Constants.cpp:12-18, Constants.h:21
+constexpr double kBohr = 0.52917721092;
+constexpr double kAutokcalpmol = 627.509474277194;
+constexpr int kSobolCols = 62;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+// One basis row: value = (f0 * f1) * f2 with f = ho_deg(q[mode]); deg 0
+// factors are 1.0.  Packed as mode (bits 0-15) | deg (bits 16-17) per
+// factor, three factors per row.
+struct RowDesc {
+    uint32_t f[3];
+};
+static inline uint32_t pack_factor(int mode, int deg)
+{
+    return (uint32_t)mode | ((uint32_t)deg << 16);
+}
+
+// Per-kernel HIP-event timing on the context's stream.
+struct KernelTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+};
+
+}  // namespace scp3g
+
+struct scp3g_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // system
+    int nw = 0, nmodes0 = 0;
+    std::vector<double> x0, omega, MUa;
+    // basis
+    std::vector<int> modes;
+    int nmodes2 = 0, nmodes3 = 0;
+    int64_t nbasis = 0;
+    std::vector<scp3g::RowDesc> desc;
+    int pes = SCP3G_PES_QTIP4PF;
+    int precision = SCP3G_FP64;
+    int tile = 4096;
+
+    // device copies of the set-up
+    uint64_t *d_sobol = nullptr;  // nmodes0 x 62
+    double *d_x0 = nullptr, *d_omega = nullptr, *d_MUa = nullptr;
+    int *d_modes = nullptr;
+    scp3g::RowDesc *d_desc = nullptr;
+
+    // per-tile work buffers (tile points)
+    int work_tile = 0;
+    int64_t work_n = 0;
+    double *d_y = nullptr;     // nmodes0 x tile
+    void *d_crd = nullptr;     // 9nw x tile (double)
+    double *d_E = nullptr;     // tile
+    double *d_w = nullptr;     // tile
+    void *d_B = nullptr;       // n x tile (double or float)
+
+    // running sums
+    double *d_H = nullptr;     // n x n column-major, upper
+    double *d_Hc = nullptr;    // compensation (mixed mode)
+    double *d_part = nullptr;  // split-K partials
+    size_t part_bytes = 0;
+    int64_t h_n = 0;
+    int64_t points_done = 0;
+
+    // timing
+    bool timing = false;
+    std::map<std::string, scp3g::KernelTimer> timers;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace scp3g {
+
+int fail(scp3g_ctx *ctx, int code, const std::string &msg);
+int check_hip(scp3g_ctx *ctx, hipError_t e, const char *what);
+
+// timing helpers: bracket a launch on ctx->stream
+void timer_begin(scp3g_ctx *ctx, const char *name, hipEvent_t *ev0);
+void timer_end(scp3g_ctx *ctx, const char *name, hipEvent_t ev0);
+
+// kernel launchers (defined in the .hip files); all enqueue on ctx->stream
+hipError_t launch_sobol(scp3g_ctx *ctx, const uint64_t *dirs, int dims, int64_t skip,
+                        int64_t npts, int halve, double *y, int64_t ldy, uint64_t *ints);
+hipError_t launch_map(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts,
+                      void *crd, int64_t ldc, int fp32);
+hipError_t launch_pes(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const void *crd,
+                      int64_t ldc, int fp32, double *E);
+hipError_t launch_weight(scp3g_ctx *ctx, const double *E, const double *y, int64_t ldy,
+                         int64_t npts, double *w);
+hipError_t launch_basis(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts, void *B,
+                        int64_t ldb, int fp32);
+hipError_t launch_syrk(scp3g_ctx *ctx, int mode, int64_t n, const void *B, int64_t ldb,
+                       const double *w, int64_t npts, double *H, double *Hc);
+hipError_t launch_finalize(scp3g_ctx *ctx, const double *H, int64_t n, double scale,
+                           const double *diag_add, double *out);
+
+// host helpers
+std::vector<double> ho_diagonal(const scp3g_ctx *ctx);  // HO diagonal + unselected ZPE
+void sobol_directions(int dims, uint64_t *v);  // dims x 62, scaled
+int jacobi_eigh(int n, std::vector<double> &A, std::vector<double> &w,
+                std::vector<double> &V);
+
+}  // namespace scp3g

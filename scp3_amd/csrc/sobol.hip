@@ -1,0 +1,129 @@
+// Sobol sampling + Beasley-Springer-Moro inverse normal on the GPU.
+//
+// Reference: sobol::std_normal (sobol.cpp:28706-28714) = i8_sobol
+// (sobol.cpp:14198-27867) followed by beasley_springer_moro
+// (beasley_springer_moro.cpp:14-46), then SCP3.cpp:201 divides by sqrt(2).
+//
+// The reference advances one point per call with the Antonov-Saleev update
+// lastq ^= v[lo0(seed)-1].  Here each lane owns one point of a 64-point
+// group and a wave steps the whole group by 64 seeds at a time: adding 2^6
+// to s flips, in the Gray code s ^ (s >> 1), exactly bit 5 and bit
+// 6 + ctz(~(s >> 6)), so the state advances by two XORs.  The starting
+// state of each lane is the XOR of v[j] over the set bits of its Gray code,
+// which equals the reference's catch-up sum.  The integers are therefore
+// bit-identical to i8_sobol's lastq; the normals match the reference up to
+// the device log() used in BSM's tail branch.
+#include "scp3g_internal.h"
+
+#include "sobol_dirnums.inc"
+
+namespace scp3g {
+
+// Direction integers scaled by 2^(61-j) (sobol.cpp:27700-27780).
+void sobol_directions(int dims, uint64_t *v)
+{
+    for (int d = 0; d < dims; d++) {
+        uint64_t *row = v + (size_t)d * kSobolCols;
+        if (d == 0) {
+            for (int j = 0; j < kSobolCols; j++) row[j] = 1;
+        } else {
+            const unsigned p = scp3_sobol_poly[d];
+            const int m = 31 - __builtin_clz(p);  // degree of the polynomial
+            for (int j = 0; j < m; j++) row[j] = scp3_sobol_init[d][j];
+            for (int j = m; j < kSobolCols; j++) {
+                // Bratley-Fox recurrence: coefficient a_k of x^(m-1-k) is
+                // bit (m-1-k) of p; term scaled by 2^(k+1)
+                uint64_t nv = row[j - m];
+                for (int k = 0; k < m; k++)
+                    if ((p >> (m - 1 - k)) & 1u) nv ^= row[j - k - 1] << (k + 1);
+                row[j] = nv;
+            }
+        }
+        for (int j = 0; j < kSobolCols; j++) row[j] <<= (kSobolCols - 1 - j);
+    }
+}
+
+// beasley_springer_moro.cpp:18-23
+__constant__ double kBsmA[4] = {2.50662823884, -18.61500062529, 41.39119773534,
+                                -25.44106049637};
+__constant__ double kBsmB[4] = {-8.47351093090, 23.08336743743, -21.06224101826,
+                                3.13082909833};
+__constant__ double kBsmC[9] = {0.3374754822726147, 0.9761690190917186, 0.1607979714918209,
+                                0.0276438810333863, 0.0038405729373609, 0.0003951896511919,
+                                0.0000321767881768, 0.0000002888167364, 0.0000003960315187};
+
+// Same operation order as beasley_springer_moro.cpp:25-44, no contraction.
+__device__ __forceinline__ double bsm(double u)
+{
+#pragma clang fp contract(off)
+    const double yj = u - 0.5;
+    double x, r;
+    if (fabs(yj) < 0.42) {
+        r = yj * yj;
+        x = yj * (((kBsmA[3] * r + kBsmA[2]) * r + kBsmA[1]) * r + kBsmA[0]) /
+            ((((kBsmB[3] * r + kBsmB[2]) * r + kBsmB[1]) * r + kBsmB[0]) * r + 1);
+    } else {
+        r = (yj > 0) ? log(-log(1 - u)) : log(-log(u));
+        x = kBsmC[0] +
+            r * (kBsmC[1] +
+                 r * (kBsmC[2] +
+                      r * (kBsmC[3] +
+                           r * (kBsmC[4] +
+                                r * (kBsmC[5] + r * (kBsmC[6] + r * (kBsmC[7] + r * kBsmC[8])))))));
+        if (yj < 0) x = -x;
+    }
+    return x;
+}
+
+constexpr int kSobolPPT = 16;  // points per lane (strided by 64)
+
+// grid: x = groups of (waves per block * 64 * PPT) points, y = dimension
+__global__ void __launch_bounds__(256)
+    k_sobol(const uint64_t *__restrict__ dirs, int64_t skip, int64_t npts, int halve,
+            double *__restrict__ y, int64_t ldy, uint64_t *__restrict__ ints)
+{
+#pragma clang fp contract(off)
+    const int d = blockIdx.y;
+    const uint64_t *__restrict__ v = dirs + (size_t)d * kSobolCols;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t i0 =
+        ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * (64 * kSobolPPT) + lane;
+    if (i0 >= npts) return;
+    uint64_t s = (uint64_t)(skip + i0);
+    uint64_t g = s ^ (s >> 1);
+    uint64_t st = 0;
+    while (g) {
+        st ^= v[__builtin_ctzll(g)];
+        g &= g - 1;
+    }
+    const uint64_t v5 = v[5];
+    double *__restrict__ yd = y + (size_t)d * ldy;
+    uint64_t *__restrict__ id = ints ? ints + (size_t)d * ldy : nullptr;
+    for (int k = 0; k < kSobolPPT; ++k) {
+        const int64_t i = i0 + 64 * k;
+        if (i >= npts) break;
+        const double u = (double)st * 0x1p-62;  // recipd, sobol.cpp:27780,27846
+        double x = bsm(u);
+        if (halve) x = x / 1.4142135623730951;  // by /= sqrt(2.0), SCP3.cpp:201
+        yd[i] = x;
+        if (id) id[i] = st;
+        const int c = __builtin_ctzll(~(s >> 6));
+        st ^= v5 ^ v[6 + c];
+        s += 64;
+    }
+}
+
+hipError_t launch_sobol(scp3g_ctx *ctx, const uint64_t *dirs, int dims, int64_t skip,
+                        int64_t npts, int halve, double *y, int64_t ldy, uint64_t *ints)
+{
+    if (npts <= 0) return hipSuccess;
+    const int waves = 4;
+    const int64_t per_block = (int64_t)waves * 64 * kSobolPPT;
+    dim3 grid((unsigned)((npts + per_block - 1) / per_block), (unsigned)dims);
+    hipLaunchKernelGGL(k_sobol, grid, dim3(64 * waves), 0, ctx->stream, dirs, skip, npts,
+                       halve, y, ldy, ints);
+    return hipGetLastError();
+}
+
+}  // namespace scp3g
