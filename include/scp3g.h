@@ -123,10 +123,25 @@ int scp3g_h_download(scp3g_ctx *ctx, double *M_host, int raw);
  * Lowest k eigenvalues (hartree, ascending) of Mout, computed on the GPU in
  * FP64 (replaces eigensolver.cpp:20-80 dsyevr RANGE='I').  evecs_host may be
  * NULL; otherwise n x k column-major.  tol is the residual tolerance in
- * hartree (<= 0 selects 1e-9).
+ * hartree (<= 0 selects 1e-9); k <= 28.
  */
 int scp3g_lowest_levels(scp3g_ctx *ctx, int k, double tol, double *evals_host,
                         double *evecs_host);
+
+/*
+ * Lowest k eigenvalues of the leading nsub x nsub block of Mout (the E0
+ * monitor of SCP3.cpp:241-247 uses the singles and singles+doubles blocks).
+ */
+int scp3g_lowest_levels_sub(scp3g_ctx *ctx, int64_t nsub, int k, double tol, double *evals_host);
+
+/*
+ * Lowest k eigenpairs of a caller-supplied symmetric n x n matrix
+ * (column-major; the upper triangle is read unless M(0, n-1) == 0, then the
+ * lower one, as eigensolver.cpp:27-30 chooses UPLO), on the context's GPU.
+ * Replaces eigensolver.cpp's dsyevr on the matrix read from sM_*.h5.
+ */
+int scp3g_matrix_lowest_levels(scp3g_ctx *ctx, int64_t n, const double *M_host, int k, double tol,
+                               double *evals_host, double *evecs_host);
 
 /* ---- per-stage entry points (device pointers; used by parity tests) ---- */
 

@@ -49,6 +49,10 @@ _SIGS = {
     "scp3g_h_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "scp3g_lowest_levels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
                                            ctypes.c_void_p]),
+    "scp3g_lowest_levels_sub": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_void_p]),
+    "scp3g_matrix_lowest_levels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int,
+                                                  ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     "scp3g_sobol_normals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     "scp3g_map_coords": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
@@ -195,6 +199,19 @@ class Context:
         ev = np.zeros(k)
         V = np.zeros((n, k), order="F") if vectors else None
         self.check(self.L.scp3g_lowest_levels(self._h, k, tol, _ptr(ev), _ptr(V)))
+        return (ev, V) if vectors else ev
+
+    def lowest_levels_sub(self, nsub, k, tol=0.0):
+        ev = np.zeros(k)
+        self.check(self.L.scp3g_lowest_levels_sub(self._h, nsub, k, tol, _ptr(ev)))
+        return ev
+
+    def matrix_lowest_levels(self, M, k, tol=0.0, vectors=False):
+        A = np.asfortranarray(M, dtype=np.float64)
+        n = A.shape[0]
+        ev = np.zeros(k)
+        V = np.zeros((n, k), order="F") if vectors else None
+        self.check(self.L.scp3g_matrix_lowest_levels(self._h, n, _ptr(A), k, tol, _ptr(ev), _ptr(V)))
         return (ev, V) if vectors else ev
 
     def sync(self):

@@ -64,7 +64,42 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_cli(verbose)
     return LIB
+
+
+HDF5_INC = os.environ.get("SCP3G_HDF5_INC", "/opt/conda/include")
+HDF5_LIB = os.environ.get("SCP3G_HDF5_LIB", "/opt/conda/lib")
+BIN = os.path.join(HERE, "bin")
+
+
+def build_cli(verbose=False):
+    """The SCP3 and eigensolver drivers, linked against libscp3g.so (and the
+    HDF5 C library for sM_*.h5 when its header is present)."""
+    os.makedirs(BIN, exist_ok=True)
+    hdf5 = os.path.exists(os.path.join(HDF5_INC, "hdf5.h"))
+    extra = []
+    if hdf5:
+        # the HDF5 directory also holds an older libstdc++ than the ROCm
+        # toolchain needs, so the driver's run path gets a directory with a
+        # link to libhdf5 alone
+        real = os.path.realpath(os.path.join(HDF5_LIB, "libhdf5.so"))
+        soname = "libhdf5.so." + real.split("libhdf5.so.")[1].split(".")[0]
+        ldir = os.path.join(BIN, "hdf5")
+        os.makedirs(ldir, exist_ok=True)
+        link = os.path.join(ldir, soname)
+        if os.path.lexists(link):
+            os.remove(link)
+        os.symlink(real, link)
+        extra = ["-DSCP3G_HAVE_HDF5", "-I" + HDF5_INC, real, "-Wl,-rpath,$ORIGIN/hdf5"]
+    for src, exe in (("scp3_main.cpp", "SCP3"), ("eigensolver_main.cpp", "eigensolver")):
+        cmd = ["g++", "-O2", "-std=c++17", os.path.join(CSRC, src), "-o", os.path.join(BIN, exe),
+               "-L" + HERE, "-lscp3g", "-Wl,-rpath,$ORIGIN/.."] + extra
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("CLI build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+        if verbose:
+            print("built", os.path.join(BIN, exe), "(HDF5)" if hdf5 else "(no HDF5)")
 
 
 if __name__ == "__main__":

@@ -28,7 +28,7 @@ namespace scp3g {
 // H, through 32 x 32 LDS tiles so that both the read of the mirrored
 // triangle and the write stay coalesced.
 __global__ void __launch_bounds__(256)
-    k_sym_full(const double *__restrict__ H, int64_t n, double P,
+    k_sym_full(const double *__restrict__ H, int64_t ldh, int64_t n, double P,
                const double *__restrict__ diag, double *__restrict__ A)
 {
     __shared__ double t[32][33];
@@ -39,8 +39,8 @@ __global__ void __launch_bounds__(256)
         for (int k = ty; k < 32; k += 8) {
             const int64_t r = br + tx, c = bc + k;
             if (r < n && c < n) {
-                double v = (r <= c) ? H[(size_t)c * n + r] / P : 0.0;
-                if (r > c) v = H[(size_t)r * n + c] / P;
+                double v = (r <= c) ? H[(size_t)c * ldh + r] / P : 0.0;
+                if (r > c) v = H[(size_t)r * ldh + c] / P;
                 if (r == c) v += diag[r];
                 A[(size_t)c * n + r] = v;
             }
@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256)
         // (rows bc.., cols br..) coalesced, transpose through LDS
         for (int k = ty; k < 32; k += 8) {
             const int64_t r = bc + tx, c = br + k;  // element H(r, c), r < c
-            t[k][tx] = (r < n && c < n) ? H[(size_t)c * n + r] / P : 0.0;
+            t[k][tx] = (r < n && c < n) ? H[(size_t)c * ldh + r] / P : 0.0;
         }
         __syncthreads();
         for (int k = ty; k < 32; k += 8) {
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(256)
 // A block owns 64 output rows r; it streams c in chunks of 64 through LDS.
 constexpr int kMvRows = 64;
 constexpr int kMvChunk = 64;
-constexpr int kMaxB = 16;
+constexpr int kMaxB = 32;
 
 __global__ void __launch_bounds__(256)
     k_symm_block(const double *__restrict__ A, int64_t n, const double *__restrict__ X,
@@ -152,6 +152,12 @@ __global__ void __launch_bounds__(256)
     for (int k = 0; k < p; ++k) acc = fma(Pm[(size_t)k * ldp + r], S[(size_t)j * lds + k], acc);
     double *yp = Y + (size_t)j * ldy + r;
     *yp = (beta == 0.0 ? 0.0 : beta * *yp) + alpha * acc;
+}
+
+__global__ void k_diag_of(const double *__restrict__ A, int64_t n, double *__restrict__ d)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r < n) d[r] = A[(size_t)r * n + r];
 }
 
 // R = AX - X theta; T = R / (theta - diag) (Davidson correction).
@@ -244,22 +250,17 @@ struct Dav {
 
 }  // namespace
 
-extern "C" int scp3g_lowest_levels(scp3g_ctx *ctx, int k, double tol, double *evals,
-                                   double *evecs)
+// Lowest k eigenpairs of A = sym(Hsrc[0:n, 0:n] / P) + diag, where Hsrc is a
+// device upper triangle with leading dimension ldh.
+static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t n, double P,
+                        const std::vector<double> &dg, int k, double tol, double *evals,
+                        double *evecs)
 {
-    if (!ctx) return -1;
-    if (ctx->nbasis <= 0 || !ctx->d_H) return fail(ctx, -2, "no Hamiltonian accumulated");
-    if (ctx->points_done <= 0) return fail(ctx, -6, "no points accumulated");
-    const int64_t n = ctx->nbasis;
-    if (k < 1 || k > 12 || k > n) return fail(ctx, -4, "k must be in 1..min(12, n)");
+    if (k < 1 || k > 28 || k > n) return fail(ctx, -4, "k must be in 1..min(28, n)");
     if (!evals) return fail(ctx, -4, "null output");
     if (tol <= 0) tol = 1e-9;
     hipSetDevice(ctx->device);
     int rc;
-
-    // HO diagonal, same as scp3g_h_download
-    std::vector<double> dg = ho_diagonal(ctx);
-
     const int b = std::min<int64_t>(k + 4, std::min<int64_t>(kMaxB, n));
     const int mmax = (int)std::min<int64_t>(n, std::max(4 * b, 64));
     const size_t nn_ = (size_t)n;
@@ -307,16 +308,23 @@ extern "C" int scp3g_lowest_levels(scp3g_ctx *ctx, int k, double tol, double *ev
         hipEvent_t ev;
         timer_begin(ctx, "symfill", &ev);
         dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
-        hipLaunchKernelGGL(k_sym_full, grid, dim3(256), 0, ctx->stream, ctx->d_H, n,
-                           (double)ctx->points_done, dv.diag, d_A);
+        hipLaunchKernelGGL(k_sym_full, grid, dim3(256), 0, ctx->stream, Hsrc, ldh, n, P, dv.diag,
+                           d_A);
         timer_end(ctx, "symfill", ev);
         DV_CHECK(hipGetLastError(), "symfill");
+        // the preconditioner and the start vectors use the diagonal of A itself
+        hipLaunchKernelGGL(k_diag_of, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                           d_A, n, dv.diag);
+        DV_CHECK(hipGetLastError(), "diag");
     }
+    std::vector<double> dA(nn_);
+    DV_CHECK(hipMemcpyAsync(dA.data(), dv.diag, nn_ * 8, hipMemcpyDeviceToHost, ctx->stream), "diag");
+    DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
     // start: unit vectors on the b lowest diagonal entries
     std::vector<int64_t> order(nn_);
     std::iota(order.begin(), order.end(), 0);
     std::partial_sort(order.begin(), order.begin() + b, order.end(),
-                      [&](int64_t x, int64_t y) { return dg[x] < dg[y]; });
+                      [&](int64_t x, int64_t y) { return dA[x] < dA[y]; });
     {
         std::vector<double> V0(nn_ * b, 0.0);
         for (int j = 0; j < b; ++j) V0[(size_t)j * n + order[j]] = 1.0;
@@ -438,4 +446,49 @@ extern "C" int scp3g_lowest_levels(scp3g_ctx *ctx, int k, double tol, double *ev
     cleanup();
 #undef DV_CHECK
     return 0;
+}
+
+extern "C" int scp3g_lowest_levels(scp3g_ctx *ctx, int k, double tol, double *evals,
+                                   double *evecs)
+{
+    if (!ctx) return -1;
+    if (ctx->nbasis <= 0 || !ctx->d_H) return fail(ctx, -2, "no Hamiltonian accumulated");
+    if (ctx->points_done <= 0) return fail(ctx, -6, "no points accumulated");
+    return davidson_run(ctx, ctx->d_H, ctx->nbasis, ctx->nbasis, (double)ctx->points_done,
+                        ho_diagonal(ctx), k, tol, evals, evecs);
+}
+
+extern "C" int scp3g_lowest_levels_sub(scp3g_ctx *ctx, int64_t nsub, int k, double tol,
+                                       double *evals)
+{
+    if (!ctx) return -1;
+    if (ctx->nbasis <= 0 || !ctx->d_H) return fail(ctx, -2, "no Hamiltonian accumulated");
+    if (ctx->points_done <= 0) return fail(ctx, -6, "no points accumulated");
+    if (nsub < 1 || nsub > ctx->nbasis) return fail(ctx, -4, "nsub out of range");
+    std::vector<double> dg = ho_diagonal(ctx);
+    dg.resize((size_t)nsub);
+    return davidson_run(ctx, ctx->d_H, ctx->nbasis, nsub, (double)ctx->points_done, dg, k, tol,
+                        evals, nullptr);
+}
+
+extern "C" int scp3g_matrix_lowest_levels(scp3g_ctx *ctx, int64_t n, const double *M_host, int k,
+                                          double tol, double *evals, double *evecs)
+{
+    if (!ctx) return -1;
+    if (n < 1 || !M_host) return fail(ctx, -4, "bad matrix");
+    hipSetDevice(ctx->device);
+    // eigensolver.cpp:27-30: the upper triangle unless M(0, n-1) == 0, then
+    // the lower one; the device copy always holds the chosen triangle as upper
+    const bool lower = n > 1 && M_host[(size_t)(n - 1) * n] == 0.0;
+    std::vector<double> U((size_t)n * n, 0.0);
+    for (int64_t c = 0; c < n; ++c)
+        for (int64_t r = 0; r <= c; ++r)
+            U[(size_t)c * n + r] = lower ? M_host[(size_t)r * n + c] : M_host[(size_t)c * n + r];
+    double *d_M = nullptr;
+    int rc = check_hip(ctx, hipMalloc(&d_M, U.size() * 8), "alloc M");
+    if (rc) return rc;
+    rc = check_hip(ctx, hipMemcpy(d_M, U.data(), U.size() * 8, hipMemcpyHostToDevice), "upload M");
+    if (!rc) rc = davidson_run(ctx, d_M, n, n, 1.0, std::vector<double>((size_t)n, 0.0), k, tol, evals, evecs);
+    hipFree(d_M);
+    return rc;
 }

@@ -63,26 +63,50 @@ def _sobol(ref):
 
 
 def _ps(ref):
-    """Coefficient arrays of ps.cpp as data (name -> list of numbers)."""
+    """Coefficient and exponent tables of ps.cpp as data.
+
+    c5z is combined as the reference does at first use (ps.cpp:507-512),
+    ((f5z*c5zA + fbasis*cbasis) + fcore*ccore) + frest*crest, in IEEE double
+    (Python floats), so the values are bit-identical.
+    """
     src = open(os.path.join(ref, "ps.cpp")).read()
     arrays = {}
-    pat = re.compile(r"(?:static\s+)?(?:const\s+)?(double|int)\s+(\w+)\s*\[([^\]]*)\](?:\s*\[([^\]]*)\])?\s*=\s*\{(.*?)\};", re.S)
-    for typ, name, d1, d2, body in pat.findall(src):
-        vals = []
-        for tok in re.split(r"[,\s{}]+", body):
-            tok = tok.strip()
-            if not tok:
-                continue
-            vals.append(tok)
-        arrays[name] = (typ, d1, d2, vals)
-    out = ["// Generated by gen_tables.py from the reference ps.cpp (Partridge-Schwenke",
-           "// monomer potential and dipole-moment-surface coefficients, used by TTM3-F)."]
-    for name, (typ, d1, d2, vals) in arrays.items():
-        ctype = "double" if typ == "double" else "int"
-        out.append("static const %s scp3_ps_%s[%d] = {" % (ctype, name, len(vals)))
-        for k in range(0, len(vals), 6):
-            out.append("  " + ", ".join(vals[k:k + 6]) + ",")
+    pat = re.compile(r"const\s+(double|size_t)\s+(\w+)\[(\d+)\]\s*=\s*\{(.*?)\};", re.S)
+    for typ, name, n, body in pat.findall(src):
+        vals = [t.strip() for t in body.replace("\n", " ").split(",") if t.strip()]
+        assert len(vals) == int(n), (name, len(vals), n)
+        arrays[name] = (typ, vals)
+    sc = {}
+    for nm in ("f5z", "fbasis", "fcore", "frest"):
+        sc[nm] = float(re.search(r"const double %s = ([-0-9.eE+]+);" % nm, src).group(1))
+    A = [float(v) for v in arrays["c5zA"][1]]
+    Bb = [float(v) for v in arrays["cbasis"][1]]
+    Cc = [float(v) for v in arrays["ccore"][1]]
+    Rr = [float(v) for v in arrays["crest"][1]]
+    c5z = [sc["f5z"] * A[i] + sc["fbasis"] * Bb[i] + sc["fcore"] * Cc[i] + sc["frest"] * Rr[i]
+           for i in range(245)]
+    out = ["// Generated by gen_tables.py from the reference ps.cpp: Partridge-Schwenke",
+           "// monomer potential (c5z = f5z*c5zA + fbasis*cbasis + fcore*ccore + frest*crest,",
+           "// ps.cpp:507-512, exponent tables idx1..3) and dipole surface (coefD, idxD0..2).",
+           "// SCP3_PS_STORAGE lets C++/HIP code make them constexpr.",
+           "#ifndef SCP3_PS_STORAGE",
+           "#define SCP3_PS_STORAGE static const",
+           "#endif"]
+    out.append("SCP3_PS_STORAGE double scp3_ps_c5z[245] = {")
+    for k in range(0, 245, 4):
+        out.append("  " + ", ".join(repr(v) for v in c5z[k:k + 4]) + ",")
+    out.append("};")
+    for name in ("idx1", "idx2", "idx3", "idxD0", "idxD1", "idxD2"):
+        vals = arrays[name][1]
+        out.append("SCP3_PS_STORAGE unsigned char scp3_ps_%s[%d] = {" % (name, len(vals)))
+        for k in range(0, len(vals), 20):
+            out.append("  " + ", ".join(vals[k:k + 20]) + ",")
         out.append("};")
+    vals = arrays["coefD"][1]
+    out.append("SCP3_PS_STORAGE double scp3_ps_coefD[84] = {")
+    for k in range(0, 84, 3):
+        out.append("  " + ", ".join(vals[k:k + 3]) + ",")
+    out.append("};")
     return "\n".join(out) + "\n"
 
 

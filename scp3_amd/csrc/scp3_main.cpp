@@ -1,0 +1,234 @@
+// SCP3 command-line driver over libscp3g (the -G estimator on the GPU).
+//
+// Options and outputs follow SCP3.cpp:33-151,274-323:
+//   SCP3 [-N p] [-S p] [-r file] [-2 n] [-3 n] [-p pes] [-m list] [-G] [-B w] input
+//   -N/--NSobol p       2^p Sobol points (default 2^20)
+//   -S/--skip p         Sobol skip 2^p (default 2^30)
+//   -r/--rng-file f     standard normals, nmodes values per point, instead of Sobol
+//   -2/--doubles n      modes with double excitations (default 3N-6)
+//   -3/--triples n      modes with triple excitations (default 0)
+//   -p/--PES name       qtip4pf (default) or ttm3f
+//   -m/--select-modes   "a-b,c" list of modes (default all)
+//   -G/--no-gradient    the B^T diag(V) B estimator (required: the gradient
+//                       estimator has no GPU path)
+//   -B/--block-width w  points per block between checkpoints (default 1024)
+// Additions: --precision fp64|mixed (default fp64), --levels k (lowest k
+// levels printed at the end, default 10, 0 = none), --device d.
+// Files: omega0.dat, E0_sd.dat (every 2^14 points), E0_t.dat (empty),
+// sM_%07d.h5 (every 2^17 points), levels.dat (this driver's final levels).
+#include <getopt.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <iomanip>
+#include <vector>
+
+#include "../../include/scp3g.h"
+#include "cli_common.h"
+
+namespace {
+
+struct Opts {
+    int64_t nsobol = (int64_t)1 << 20;
+    int64_t skip = (int64_t)1 << 30;
+    std::string rng_file, pes = "qtip4pf", input, modes_desc;
+    int n2 = 0, n3 = 0, bw = 1024, levels = 10, device = 0;
+    bool gradient = true, have_modes = false;
+    int precision = SCP3G_FP64;
+};
+
+// parse_mode_description (SCP3.cpp:63-89): "a-b,c" -> a..b, c
+std::vector<int> parse_modes(const std::string &desc)
+{
+    std::vector<int> out;
+    std::istringstream iss(desc);
+    while (iss.good()) {
+        if (iss.peek() == ',') iss.get();
+        int start, stop;
+        if (!(iss >> start)) break;
+        stop = start;
+        if (iss.peek() == '-') {
+            iss.get();
+            iss >> stop;
+        }
+        for (; start <= stop; ++start) out.push_back(start);
+    }
+    return out;
+}
+
+[[noreturn]] void die(const std::string &msg)
+{
+    std::cerr << msg << "\n";
+    std::exit(EXIT_FAILURE);
+}
+
+Opts parse(int argc, char **argv)
+{
+    static option lo[] = {{"NSobol", required_argument, nullptr, 'N'},
+                          {"skip", required_argument, nullptr, 'S'},
+                          {"rng-file", required_argument, nullptr, 'r'},
+                          {"doubles", required_argument, nullptr, '2'},
+                          {"triples", required_argument, nullptr, '3'},
+                          {"PES", required_argument, nullptr, 'p'},
+                          {"select-modes", required_argument, nullptr, 'm'},
+                          {"no-gradient", no_argument, nullptr, 'G'},
+                          {"block-width", required_argument, nullptr, 'B'},
+                          {"precision", required_argument, nullptr, 1000},
+                          {"levels", required_argument, nullptr, 1001},
+                          {"device", required_argument, nullptr, 1002},
+                          {nullptr, 0, nullptr, 0}};
+    Opts o;
+    int ch;
+    while ((ch = getopt_long(argc, argv, "S:N:2:3:r:p:m:GB:", lo, nullptr)) != -1) {
+        int p2;
+        switch (ch) {
+        case 'N':
+            p2 = atoi(optarg);
+            // the reference stops at 32 and overflows an int shift at 31, 32
+            if (p2 < 0 || p2 > 40) die("-N <N>: <N>is too large, NSobol = 2^<N>");
+            o.nsobol = (int64_t)1 << p2;
+            break;
+        case 'S':
+            p2 = atoi(optarg);
+            if (p2 < 0 || p2 > 50) die("-S <N>: <N>is too large, sobol_skip = 2^<N>");
+            o.skip = (int64_t)1 << p2;
+            break;
+        case '2': o.n2 = atoi(optarg); break;
+        case '3': o.n3 = atoi(optarg); break;
+        case 'r': o.rng_file = optarg; break;
+        case 'p': o.pes = optarg; break;
+        case 'm':
+            o.modes_desc = optarg;
+            o.have_modes = true;
+            break;
+        case 'G': o.gradient = false; break;
+        case 'B': o.bw = atoi(optarg); break;
+        case 1000:
+            if (!strcmp(optarg, "fp64")) o.precision = SCP3G_FP64;
+            else if (!strcmp(optarg, "mixed")) o.precision = SCP3G_MIXED;
+            else die("--precision must be fp64 or mixed");
+            break;
+        case 1001: o.levels = atoi(optarg); break;
+        case 1002: o.device = atoi(optarg); break;
+        default: die("Unknown option: " + std::to_string(ch));
+        }
+    }
+    if (optind >= argc) die("No input file supplied");
+    o.input = argv[optind];
+    if (o.bw <= 0) die("-B <w>: block width must be positive");
+    return o;
+}
+
+void check(scp3g_ctx *ctx, int rc, const char *what)
+{
+    if (rc != 0) die(std::string(what) + ": " + scp3g_last_error(ctx));
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    Opts o = parse(argc, argv);
+    if (o.pes == "whbb" || o.pes == "hbb2-pol") die("Support for Bowman's WHBB was not included.");
+    if (o.pes != "qtip4pf" && o.pes != "ttm3f") die("Unknown H2O potential: " + o.pes);
+    if (o.gradient)
+        die("The gradient estimator (the default without -G) has no GPU path in this build; "
+            "run with -G for the B^T diag(V) B estimator.");
+
+    cli::Input in;
+    std::string err;
+    if (!cli::load_vladimir(o.input, in, err)) die(err);
+    if (!cli::ohhohh(in, err)) die(err);
+    if (in.H.empty()) die("input holds no Hessian");
+    const int n3c = in.n3, N = n3c / 3, nm0 = n3c - 6;
+    std::vector<double> omega(nm0), MUa((size_t)n3c * nm0), oall(n3c);
+    if (scp3g_normal_modes(n3c, in.mass.data(), in.H.data(), omega.data(), MUa.data(), oall.data()))
+        die("normal modes failed (imaginary frequency among the kept modes?)");
+    {
+        std::ofstream sout("omega0.dat");  // SCP3.cpp:300-302
+        sout << std::setprecision(10);
+        for (double w : oall) sout << w * cli::autocm << "\n";
+    }
+    if (o.n2 == 0) o.n2 = 3 * N - 6;  // SCP3.cpp:305
+    std::vector<int> modes;
+    if (o.have_modes) modes = parse_modes(o.modes_desc);
+    else
+        for (int k = 0; k < nm0; ++k) modes.push_back(k);
+    if (o.n2 > (int)modes.size() || o.n3 > (int)modes.size())
+        die("Number of double or triple excitations exceed number of modes.");
+
+    scp3g_ctx *ctx = nullptr;
+    if (scp3g_create(&ctx, o.device)) die("no usable GPU");
+    check(ctx, scp3g_set_system(ctx, N / 3, in.x0.data(), nm0, omega.data(), MUa.data()), "set_system");
+    check(ctx, scp3g_set_basis(ctx, (int)modes.size(), modes.data(), o.n2, o.n3), "set_basis");
+    check(ctx, scp3g_set_pes(ctx, o.pes.c_str()), "set_pes");
+    check(ctx, scp3g_set_precision(ctx, o.precision), "set_precision");
+    check(ctx, scp3g_reset(ctx), "reset");
+    int64_t n = 0, n1 = 0, n2s = 0;
+    scp3g_basis_size(ctx, &n);
+    scp3g_sub_basis_size(ctx, 1, &n1);
+    scp3g_sub_basis_size(ctx, 2, &n2s);
+    std::cerr << "SCP3 (GPU, -G): " << N / 3 << " waters, " << modes.size() << " modes, " << n
+              << " states, " << o.nsobol << " points, " << (o.precision ? "mixed" : "fp64")
+              << " form, " << o.pes << "\n";
+
+    std::ofstream e0sd("E0_sd.dat"), e0t("E0_t.dat");
+    e0sd << std::fixed << std::setprecision(10);
+    std::ifstream rng;
+    if (!o.rng_file.empty()) {
+        rng.open(o.rng_file.c_str());
+        if (!rng.good()) die("cannot open " + o.rng_file);
+    }
+    std::vector<double> Mout;
+    // blocks of bw points as SCP3.cpp:187; consecutive blocks without a
+    // checkpoint are sent to the device together
+    const int64_t log_every = (int64_t)1 << 14, save_every = (int64_t)1 << 17;
+    int64_t i = 0;
+    while (i < o.nsobol) {
+        int64_t span = 0;
+        do {
+            span += o.bw;
+        } while (i + span < o.nsobol && (i + span) % log_every != 0);
+        if (rng.is_open()) {
+            std::vector<double> nr((size_t)span * nm0);
+            for (double &v : nr) rng >> v;
+            if (!rng) die("rng-file ended early");
+            check(ctx, scp3g_accumulate_normals(ctx, nr.data(), span), "accumulate");
+        } else {
+            check(ctx, scp3g_accumulate(ctx, o.skip + i, span), "accumulate");
+        }
+        i += span;
+        if (i % log_every == 0) {
+            double e1, e2;
+            check(ctx, scp3g_lowest_levels_sub(ctx, n1, 1, 0.0, &e1), "E0 singles");
+            check(ctx, scp3g_lowest_levels_sub(ctx, n2s, 1, 0.0, &e2), "E0 doubles");
+            e1 *= cli::autocm;
+            e2 *= cli::autocm;
+            e0sd << i << " " << e1 << " " << e2 << " " << e2 - e1 << std::endl;
+            if (i % save_every == 0) {
+#ifdef SCP3G_HAVE_HDF5
+                Mout.resize((size_t)n * n);
+                check(ctx, scp3g_h_download(ctx, Mout.data(), 0), "download");
+                char s[128];
+                snprintf(s, sizeof(s), "sM_%07lld.h5", (long long)i);
+                if (!cli::save_hdf5(s, Mout.data(), n, err)) die(err);
+#endif
+            }
+        }
+    }
+    if (o.levels > 0) {
+        const int k = (int)std::min<int64_t>(o.levels, n);
+        std::vector<double> ev(k);
+        check(ctx, scp3g_lowest_levels(ctx, k, 0.0, ev.data(), nullptr), "levels");
+        std::ofstream lv("levels.dat");
+        lv << std::setprecision(15);
+        lv << "# lowest " << k << " levels: hartree, cm^-1, cm^-1 above the lowest\n";
+        for (int j = 0; j < k; ++j)
+            lv << ev[j] << " " << ev[j] * cli::autocm << " " << (ev[j] - ev[0]) * cli::autocm << "\n";
+        for (int j = 0; j < k; ++j) std::cout << std::setprecision(10) << ev[j] * cli::autocm << "\n";
+    }
+    scp3g_destroy(ctx);
+    return 0;
+}

@@ -25,7 +25,10 @@ def dev(a, dtype=None):
 
 
 def vp(t):
+    """Device pointer for the C ABI.  The library works on its own stream,
+    which does not wait for torch's: finish torch's pending work first."""
     import ctypes
+    torch.cuda.synchronize()
     return ctypes.c_void_p(t.data_ptr())
 
 
