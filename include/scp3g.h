@@ -143,6 +143,39 @@ int scp3g_lowest_levels_sub(scp3g_ctx *ctx, int64_t nsub, int k, double tol, dou
 int scp3g_matrix_lowest_levels(scp3g_ctx *ctx, int64_t n, const double *M_host, int k, double tol,
                                double *evals_host, double *evecs_host);
 
+/* ---- MLP surrogate and control variate --------------------------------
+ * No counterpart in the reference.  A network 3 x (width `hidden`, SiLU) on
+ * the normal coordinates (x = sqrt(2) y, padded to a multiple of 64) models
+ * V - V0 - 1/2 sum omega y^2 (normalised); BF16 weights and activations on
+ * MFMA with FP32 accumulation, FP32 master weights and Adam, all on the GPU.
+ * Parameter vector layout (FP32): [W0 (h x dpad) | b0 | W1 (h x h) | b1 |
+ * W2 (h x h) | b2 | w3 (h) | b3], W row-major [out][in].  norm[3] holds
+ * V0 (hartree), the target mean and the target scale.
+ */
+int scp3g_mlp_init(scp3g_ctx *ctx, int hidden, uint64_t seed);
+int scp3g_mlp_param_count(const scp3g_ctx *ctx, int64_t *n);
+int scp3g_mlp_get_params(scp3g_ctx *ctx, float *P_host, double *norm_host);
+int scp3g_mlp_set_params(scp3g_ctx *ctx, const float *P_host, const double *norm_host);
+/* training set from host data: y_host[npts x nmodes0] (halved normals,
+ * point-major), t_host[npts] normalised targets; npts a multiple of 64 */
+int scp3g_mlp_load_data(scp3g_ctx *ctx, const double *y_host, const float *t_host, int64_t npts);
+/* training set sampled by the path itself: Sobol seeds skip.., FP64 potential */
+int scp3g_mlp_sample_data(scp3g_ctx *ctx, int64_t skip, int64_t npts);
+/* `steps` Adam steps on consecutive batches; loss_host[steps] (nullable) */
+int scp3g_mlp_train(scp3g_ctx *ctx, int steps, int batch, double lr, float *loss_host);
+/* loss gradient over the first `batch` training points, no update */
+int scp3g_mlp_grad(scp3g_ctx *ctx, int batch, float *grad_host, float *loss_host);
+/* V_nn (hartree) of npts points, y_dev mode-major with leading dimension ldy */
+int scp3g_mlp_eval(scp3g_ctx *ctx, const double *y_dev, int64_t ldy, int64_t npts, double *V_dev);
+/*
+ * Control-variate accumulation: (1/P_nn) sum B B^T w_nn over seeds skip_nn..
+ * plus (1/P_t) sum B B^T (bV - w_nn) over seeds skip_t.. (true potential);
+ * unbiased for the same H as scp3g_accumulate.  After a reset, the running
+ * sum is normalised by the accumulated weight (here 1).
+ */
+int scp3g_accumulate_cv(scp3g_ctx *ctx, int64_t skip_nn, int64_t P_nn, int64_t skip_t,
+                        int64_t P_t);
+
 /* ---- per-stage entry points (device pointers; used by parity tests) ---- */
 
 /*

@@ -68,6 +68,19 @@ _SIGS = {
                                           ctypes.c_void_p, ctypes.c_void_p]),
     "scp3g_pes_cpu": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p]),
+    "scp3g_mlp_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64]),
+    "scp3g_mlp_param_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "scp3g_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "scp3g_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "scp3g_mlp_load_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "scp3g_mlp_sample_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
+    "scp3g_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_void_p]),
+    "scp3g_mlp_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "scp3g_mlp_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.c_void_p]),
+    "scp3g_accumulate_cv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                           ctypes.c_int64]),
     "scp3g_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_int64)]),
@@ -216,6 +229,47 @@ class Context:
 
     def sync(self):
         self.check(self.L.scp3g_sync(self._h))
+
+    # surrogate / control variate -------------------------------------------
+    def mlp_init(self, hidden, seed=0):
+        self.check(self.L.scp3g_mlp_init(self._h, hidden, seed))
+
+    def mlp_params(self):
+        n = ctypes.c_int64()
+        self.check(self.L.scp3g_mlp_param_count(self._h, ctypes.byref(n)))
+        P = np.zeros(n.value, dtype=np.float32)
+        nrm = np.zeros(3)
+        self.check(self.L.scp3g_mlp_get_params(self._h, _ptr(P), _ptr(nrm)))
+        return P, nrm
+
+    def mlp_set_params(self, P, nrm=None):
+        P = np.ascontiguousarray(P, dtype=np.float32)
+        nr = np.ascontiguousarray(nrm, dtype=np.float64) if nrm is not None else None
+        self.check(self.L.scp3g_mlp_set_params(self._h, _ptr(P), _ptr(nr)))
+
+    def mlp_load_data(self, y, t):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        self.check(self.L.scp3g_mlp_load_data(self._h, _ptr(y), _ptr(t), y.shape[0]))
+
+    def mlp_sample_data(self, skip, npts):
+        self.check(self.L.scp3g_mlp_sample_data(self._h, int(skip), int(npts)))
+
+    def mlp_train(self, steps, batch, lr):
+        loss = np.zeros(steps, dtype=np.float32)
+        self.check(self.L.scp3g_mlp_train(self._h, steps, batch, lr, _ptr(loss)))
+        return loss
+
+    def mlp_grad(self, batch):
+        n = ctypes.c_int64()
+        self.check(self.L.scp3g_mlp_param_count(self._h, ctypes.byref(n)))
+        g = np.zeros(n.value, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self.check(self.L.scp3g_mlp_grad(self._h, batch, _ptr(g), _ptr(loss)))
+        return g, float(loss[0])
+
+    def accumulate_cv(self, skip_nn, P_nn, skip_t, P_t):
+        self.check(self.L.scp3g_accumulate_cv(self._h, int(skip_nn), int(P_nn), int(skip_t), int(P_t)))
 
     # timing -------------------------------------------------------------
     def set_timing(self, on):

@@ -23,6 +23,7 @@ SOURCES = [
     "syrk.hip",
     "syrk_mixed.hip",
     "eigen.hip",
+    "mlp.hip",
     "context.cpp",
     "modes.cpp",
     "pes_cpu.cpp",

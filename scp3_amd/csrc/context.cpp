@@ -213,6 +213,7 @@ static int ensure_sums(scp3g_ctx *ctx)
         return rc;
     ctx->h_n = n;
     ctx->points_done = 0;
+    ctx->norm_sum = 0.0;
     return 0;
 }
 
@@ -268,6 +269,7 @@ void scp3g_destroy(scp3g_ctx *ctx)
     dfree(ctx->d_modes);
     dfree(ctx->d_desc);
     release_tiles(ctx);
+    mlp_release(ctx);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -421,16 +423,29 @@ int scp3g_reset(scp3g_ctx *ctx)
     if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_Hc, 0, bytes, ctx->stream), "clear Hc")))
         return rc;
     ctx->points_done = 0;
+    ctx->norm_sum = 0.0;
     return 0;
 }
 
-// one tile: y already in ctx->d_y (nmodes0 x tile, halved), np points
-static int run_tile(scp3g_ctx *ctx, int64_t np)
+// one tile: y already in ctx->d_y (nmodes0 x tile, halved), np points.
+// wmode 0: plain weights bV (SCP3.cpp:219-223); 1: surrogate weights w_nn;
+// 2: control-variate correction bV - w_nn.  Modes 1 and 2 multiply the
+// weights by `scale`; the normaliser grows by np*scale for modes 0 and 1.
+}  // extern "C"
+
+int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
 {
     const int64_t T = ctx->tile;
     const int fp32 = ctx->precision == SCP3G_MIXED;
     hipEvent_t ev;
     int rc;
+    if (wmode == 1) {
+        timer_begin(ctx, "mlp", &ev);
+        rc = check_hip(ctx, launch_mlp_weights(ctx, ctx->d_y, T, np, nullptr, 0, scale, ctx->d_w), "mlp");
+        timer_end(ctx, "mlp", ev);
+        if (rc) return rc;
+        goto basis;
+    }
     timer_begin(ctx, "map", &ev);
     rc = check_hip(ctx, launch_map(ctx, ctx->d_y, T, np, ctx->d_crd, T, fp32), "map");
     timer_end(ctx, "map", ev);
@@ -442,10 +457,17 @@ static int run_tile(scp3g_ctx *ctx, int64_t np)
         rc = check_hip(ctx, launch_qtip4pf(ctx, ctx->nw, np, ctx->d_crd, T, fp32, ctx->d_E), "pes");
     timer_end(ctx, "pes", ev);
     if (rc) return rc;
-    timer_begin(ctx, "weight", &ev);
-    rc = check_hip(ctx, launch_weight(ctx, ctx->d_E, ctx->d_y, T, np, ctx->d_w), "weight");
-    timer_end(ctx, "weight", ev);
+    if (wmode == 2) {
+        timer_begin(ctx, "mlp", &ev);
+        rc = check_hip(ctx, launch_mlp_weights(ctx, ctx->d_y, T, np, ctx->d_E, 1, scale, ctx->d_w), "mlp");
+        timer_end(ctx, "mlp", ev);
+    } else {
+        timer_begin(ctx, "weight", &ev);
+        rc = check_hip(ctx, launch_weight(ctx, ctx->d_E, ctx->d_y, T, np, ctx->d_w), "weight");
+        timer_end(ctx, "weight", ev);
+    }
     if (rc) return rc;
+basis:
     timer_begin(ctx, "basis", &ev);
     rc = check_hip(ctx, launch_basis(ctx, ctx->d_y, T, np, ctx->d_B, T, fp32), "basis");
     timer_end(ctx, "basis", ev);
@@ -457,10 +479,20 @@ static int run_tile(scp3g_ctx *ctx, int64_t np)
     timer_end(ctx, "syrk", ev);
     if (rc) return rc;
     ctx->points_done += np;
+    if (wmode == 0) ctx->norm_sum += (double)np;
+    else if (wmode == 1) ctx->norm_sum += (double)np * scale;
     return 0;
 }
 
+extern "C" {
+
 int scp3g_accumulate(scp3g_ctx *ctx, int64_t skip, int64_t npts)
+{
+    return scp3g_accumulate_mode(ctx, skip, npts, 0, 1.0);
+}
+
+// Sobol seeds skip..skip+npts-1 with weight mode wmode (see run_tile).
+int scp3g_accumulate_mode(scp3g_ctx *ctx, int64_t skip, int64_t npts, int wmode, double scale)
 {
     int rc = ready(ctx);
     if (rc) return rc;
@@ -480,7 +512,7 @@ int scp3g_accumulate(scp3g_ctx *ctx, int64_t skip, int64_t npts)
                        "sobol");
         timer_end(ctx, "sobol", ev);
         if (rc) return rc;
-        if ((rc = run_tile(ctx, np))) return rc;
+        if ((rc = run_tile(ctx, np, wmode, scale))) return rc;
     }
     return 0;
 }
@@ -506,7 +538,7 @@ int scp3g_accumulate_normals(scp3g_ctx *ctx, const double *normals, int64_t npts
         if ((rc = check_hip(ctx, hipMemcpy(ctx->d_y, yt.data(), yt.size() * 8, hipMemcpyHostToDevice),
                             "upload normals")))
             return rc;
-        if ((rc = run_tile(ctx, np))) return rc;
+        if ((rc = run_tile(ctx, np, 0, 1.0))) return rc;
     }
     return 0;
 }
@@ -542,7 +574,7 @@ int scp3g_h_download(scp3g_ctx *ctx, double *M_host, int raw)
     hipMemcpy(d_diag, dg.data(), n * 8, hipMemcpyHostToDevice);
     hipEvent_t ev;
     timer_begin(ctx, "finalize", &ev);
-    rc = check_hip(ctx, launch_finalize(ctx, ctx->d_H, n, (double)ctx->points_done, d_diag, d_out),
+    rc = check_hip(ctx, launch_finalize(ctx, ctx->d_H, n, ctx->norm_sum, d_diag, d_out),
                    "finalize");
     timer_end(ctx, "finalize", ev);
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");

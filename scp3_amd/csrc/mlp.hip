@@ -1,0 +1,807 @@
+// MLP surrogate V_nn(q) of the potential in normal coordinates, trained and
+// evaluated by this library's own kernels, used as a QMC control variate.
+//
+// Network: x = sqrt(2) y (unit-variance inputs, padded to a multiple of 64
+// with zeros) -> 3 hidden layers of width h with SiLU -> scalar.  The output
+// models the anharmonic part of the potential,
+//     V(y) = V0 + 1/2 sum_k omega_k y_k^2 + t_mean + t_std * f(x),
+// so that the control-variate weight is w_nn = V0 + t_mean + t_std f(x)
+// (SCP3.cpp:222-223 with V replaced by V_nn).
+//
+// Arithmetic: BF16 weights and activations on v_mfma_f32_16x16x32_bf16 with
+// FP32 accumulation; FP32 master weights, gradients and Adam moments.
+// GEMMs are "NT": C[m][n] = sum_k A[m][k] B[n][k] with both operands
+// k-contiguous; weights are kept as W[out][in] (forward) and W^T[in][out]
+// (backward data), activations and deltas as [batch][feature] and
+// [feature][batch] (weight gradients).  Epilogues fuse bias + SiLU
+// (forward), SiLU' (backward) and the layout copies.
+//
+// No reference counterpart: the reference has no surrogate (the learned part
+// and the control variate are new in this build).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "scp3g_internal.h"
+
+namespace scp3g {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline unsigned short f2bf(float f)
+{
+    unsigned u;
+    memcpy(&u, &f, 4);
+    const unsigned bias = 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)((u + bias) >> 16);
+}
+
+__device__ inline float bf2f(unsigned short h)
+{
+    return __uint_as_float((unsigned)h << 16);
+}
+
+__device__ inline float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+
+enum { EPI_FWD = 0, EPI_BWD = 1, EPI_WGRAD = 2 };
+
+// C = A B^T on a 64 x 64 tile per workgroup (2 x 2 waves of 32 x 32, 2 x 2
+// blocks of 16 x 16 x 32 each).  M, N multiples of 64, K multiple of 32.
+template <int EPI>
+__global__ void __launch_bounds__(256)
+    k_gemm_nt(const unsigned short *__restrict__ A, int64_t lda, const unsigned short *__restrict__ B,
+              int64_t ldb, int K, const float *__restrict__ bias, float *__restrict__ Z,
+              unsigned short *__restrict__ Out, unsigned short *__restrict__ OutT, int64_t ldo,
+              int64_t ldoT, const float *__restrict__ Zprev, float *__restrict__ G, int64_t ldg)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int64_t m0 = (int64_t)blockIdx.y * 64 + (wave >> 1) * 32;
+    const int64_t n0 = (int64_t)blockIdx.x * 64 + (wave & 1) * 32;
+    f4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < K; k += 32) {
+        bf16x8 av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+            av[a] = *reinterpret_cast<const bf16x8 *>(A + (size_t)(m0 + 16 * a + li) * lda + k + 8 * lg);
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            bv[b] = *reinterpret_cast<const bf16x8 *>(B + (size_t)(n0 + 16 * b + li) * ldb + k + 8 * lg);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    // D[i][j]: lane j + 16*(i/4), element i%4
+    const int oj = lane & 15, oi = 4 * (lane >> 4);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t m = m0 + 16 * a + oi + e, n = n0 + 16 * b + oj;
+                float v = acc[a][b][e];
+                if (EPI == EPI_FWD) {
+                    v += bias[n];
+                    Z[(size_t)m * ldo + n] = v;
+                    const unsigned short h = f2bf(v * sigm(v));
+                    Out[(size_t)m * ldo + n] = h;
+                    if (OutT) OutT[(size_t)n * ldoT + m] = h;
+                } else if (EPI == EPI_BWD) {
+                    const float z = Zprev[(size_t)m * ldo + n];
+                    const float s = sigm(z);
+                    const unsigned short h = f2bf(v * s * (1.0f + z * (1.0f - s)));
+                    Out[(size_t)m * ldo + n] = h;
+                    OutT[(size_t)n * ldoT + m] = h;
+                } else {
+                    G[(size_t)m * ldg + n] = v;
+                }
+            }
+}
+
+// x[p][k] = bf16(sqrt(2) y[k][p]) for k < M, zero up to dpad; optional x^T.
+__global__ void k_mlp_input(const double *__restrict__ y, int64_t ldy, int M, int dpad,
+                            int64_t npts, unsigned short *__restrict__ x,
+                            unsigned short *__restrict__ xT, int64_t ldxT)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    for (int k = 0; k < dpad; ++k) {
+        const float v = k < M ? (float)(1.4142135623730951 * y[(size_t)k * ldy + p]) : 0.0f;
+        const unsigned short h = f2bf(v);
+        x[(size_t)p * dpad + k] = h;
+        if (xT) xT[(size_t)k * ldxT + p] = h;
+    }
+}
+
+// out[p] = a3[p] . w3 + b3; one wave per point group.  Training mode also
+// forms the loss terms and the head's backward: dout = 2(out - t)/B,
+// delta3 = dout w3 silu'(z3) (both layouts).
+__global__ void __launch_bounds__(256)
+    k_mlp_head(const unsigned short *__restrict__ a3, const float *__restrict__ z3, int h,
+               const float *__restrict__ w3, const float *__restrict__ b3, int64_t npts,
+               const float *__restrict__ target, float inv_batch, float *__restrict__ out,
+               float *__restrict__ sqerr, float *__restrict__ dout,
+               unsigned short *__restrict__ d3, unsigned short *__restrict__ d3T, int64_t ldT)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= npts) return;
+    float s = 0.f;
+    for (int j = lane; j < h; j += 64) s += bf2f(a3[(size_t)p * h + j]) * w3[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    const float o = s + b3[0];
+    if (lane == 0) out[p] = o;
+    if (!target) return;
+    const float r = o - target[p];
+    const float g = 2.0f * r * inv_batch;
+    if (lane == 0) {
+        sqerr[p] = r * r;
+        dout[p] = g;
+    }
+    for (int j = lane; j < h; j += 64) {
+        const float z = z3[(size_t)p * h + j];
+        const float sg = sigm(z);
+        const unsigned short v = f2bf(g * w3[j] * sg * (1.0f + z * (1.0f - sg)));
+        d3[(size_t)p * h + j] = v;
+        d3T[(size_t)j * ldT + p] = v;
+    }
+}
+
+// Row sums of a [rows][cols] bf16 matrix (bias gradients from delta^T), and
+// the head's weight gradient sum_p dout[p] a3[p][j].
+__global__ void __launch_bounds__(256)
+    k_rowsum_bf16(const unsigned short *__restrict__ X, int64_t ld, int64_t cols,
+                  float *__restrict__ out)
+{
+    __shared__ float red[256];
+    const int r = blockIdx.x;
+    float s = 0.f;
+    for (int64_t c = threadIdx.x; c < cols; c += 256) s += bf2f(X[(size_t)r * ld + c]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[r] = red[0];
+}
+
+__global__ void __launch_bounds__(256)
+    k_head_grad(const unsigned short *__restrict__ a3T, int64_t ldT, const float *__restrict__ dout,
+                int64_t batch, float *__restrict__ gw3, float *__restrict__ gb3)
+{
+    __shared__ float red[256];
+    const int j = blockIdx.x;  // j == h gives the bias
+    float s = 0.f;
+    for (int64_t p = threadIdx.x; p < batch; p += 256)
+        s += dout[p] * (j < (int)gridDim.x - 1 ? bf2f(a3T[(size_t)j * ldT + p]) : 1.0f);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (j < (int)gridDim.x - 1) gw3[j] = red[0];
+        else gb3[0] = red[0];
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters.
+__global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ m,
+                       float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = G[i];
+    const float mi = b1 * m[i] + (1.f - b1) * g;
+    const float vi = b2 * v[i] + (1.f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 copies W[o][i] and W^T[i][o] of an FP32 [o][i] matrix.
+__global__ void k_to_bf16(const float *__restrict__ W, int rows, int cols,
+                          unsigned short *__restrict__ Wb, unsigned short *__restrict__ WbT)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)rows * cols) return;
+    const int r = (int)(e / cols), c = (int)(e - (int64_t)r * cols);
+    const unsigned short h = f2bf(W[e]);
+    Wb[e] = h;
+    WbT[(size_t)c * rows + r] = h;
+}
+
+// w_nn = V0 + t_mean + t_std * f  and, when V (hartree) is given, the
+// correction weight w - w_nn = V - V_nn (SCP3.cpp:222-223).
+__global__ void k_mlp_weights(const float *__restrict__ f, const double *__restrict__ y,
+                              int64_t ldy, const double *__restrict__ omega, int M, double base,
+                              double tstd, int64_t npts, int64_t npad, const double *__restrict__ E,
+                              int mode, double scale, double *__restrict__ w)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    if (i >= npts) {
+        w[i] = 0.0;
+        return;
+    }
+    const double wnn = base + tstd * (double)f[i];
+    if (mode == 0) {
+        w[i] = scale * wnn;
+        return;
+    }
+    double hsum = 0.0;
+    for (int k = 0; k < M; ++k) {
+        const double yk = y[(size_t)k * ldy + i];
+        hsum += yk * (omega[k] * yk);
+    }
+    const double bV = E[i] / kAutokcalpmol - 0.5 * hsum;
+    w[i] = scale * (bV - wnn);
+}
+
+// training targets t = (V - V0 - 1/2 sum omega y^2 - mean) / std, FP32
+__global__ void k_mlp_targets(const double *__restrict__ E, const double *__restrict__ y,
+                              int64_t ldy, const double *__restrict__ omega, int M, double V0,
+                              int64_t npts, double *__restrict__ raw)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    double hsum = 0.0;
+    for (int k = 0; k < M; ++k) {
+        const double yk = y[(size_t)k * ldy + i];
+        hsum += yk * (omega[k] * yk);
+    }
+    raw[i] = E[i] / kAutokcalpmol - 0.5 * hsum - V0;
+}
+
+__global__ void k_sum_f32(const float *__restrict__ x, int64_t n, float *__restrict__ out)
+{
+    __shared__ float red[256];
+    float s = 0.f;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += x[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct Layer {
+    int in, out;            // out x in weights (in padded to a multiple of 64)
+    int64_t w_off, b_off;   // offsets in the FP32 parameter vector
+    unsigned short *Wb = nullptr, *WbT = nullptr;
+};
+
+}  // namespace scp3g
+
+struct MlpState {
+    int M = 0, dpad = 0, h = 0;
+    scp3g::Layer L[3];
+    int64_t w3_off = 0, b3_off = 0, nparam = 0;
+    float *P = nullptr, *G = nullptr, *m = nullptr, *v = nullptr;  // device
+    int64_t step = 0;
+    double V0 = 0.0, t_mean = 0.0, t_std = 1.0;
+    // work buffers for a batch of `cap` points
+    int64_t cap = 0;
+    unsigned short *x = nullptr, *xT = nullptr, *act[3] = {}, *actT[3] = {}, *dl[3] = {},
+                   *dlT[3] = {};
+    float *z[3] = {}, *out = nullptr, *sq = nullptr, *dout = nullptr, *lossd = nullptr;
+    // training set
+    int64_t ntrain = 0;
+    unsigned short *Xtr = nullptr, *XtrT = nullptr;
+    float *Ttr = nullptr;
+};
+
+namespace scp3g {
+
+static void mfree(void *&p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+template <typename T>
+static void mfree(T *&p)
+{
+    void *q = p;
+    mfree(q);
+    p = nullptr;
+}
+
+static void free_batch(MlpState *s)
+{
+    mfree(s->x);
+    mfree(s->xT);
+    for (int l = 0; l < 3; ++l) {
+        mfree(s->act[l]);
+        mfree(s->actT[l]);
+        mfree(s->dl[l]);
+        mfree(s->dlT[l]);
+        mfree(s->z[l]);
+    }
+    mfree(s->out);
+    mfree(s->sq);
+    mfree(s->dout);
+    mfree(s->lossd);
+    s->cap = 0;
+}
+
+void mlp_release(scp3g_ctx *ctx)
+{
+    MlpState *s = ctx->mlp;
+    if (!s) return;
+    free_batch(s);
+    for (int l = 0; l < 3; ++l) {
+        mfree(s->L[l].Wb);
+        mfree(s->L[l].WbT);
+    }
+    mfree(s->P);
+    mfree(s->G);
+    mfree(s->m);
+    mfree(s->v);
+    mfree(s->Xtr);
+    mfree(s->XtrT);
+    mfree(s->Ttr);
+    delete s;
+    ctx->mlp = nullptr;
+}
+
+static hipError_t ensure_batch(MlpState *s, int64_t cap)
+{
+    if (s->cap >= cap) return hipSuccess;
+    free_batch(s);
+    const size_t B = (size_t)cap, h = (size_t)s->h;
+    hipError_t e = hipSuccess;
+    auto A = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, bytes);
+    };
+    A((void **)&s->x, B * s->dpad * 2);
+    A((void **)&s->xT, B * s->dpad * 2);
+    for (int l = 0; l < 3; ++l) {
+        A((void **)&s->act[l], B * h * 2);
+        A((void **)&s->actT[l], B * h * 2);
+        A((void **)&s->dl[l], B * h * 2);
+        A((void **)&s->dlT[l], B * h * 2);
+        A((void **)&s->z[l], B * h * 4);
+    }
+    A((void **)&s->out, B * 4);
+    A((void **)&s->sq, B * 4);
+    A((void **)&s->dout, B * 4);
+    A((void **)&s->lossd, 64 * 1024 * 4);
+    if (e == hipSuccess) s->cap = cap;
+    return e;
+}
+
+static hipError_t refresh_bf16(scp3g_ctx *ctx, MlpState *s)
+{
+    for (int l = 0; l < 3; ++l) {
+        const int64_t ne = (int64_t)s->L[l].out * s->L[l].in;
+        hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream,
+                           s->P + s->L[l].w_off, s->L[l].out, s->L[l].in, s->L[l].Wb, s->L[l].WbT);
+    }
+    return hipGetLastError();
+}
+
+// forward over `np` points (np a multiple of 64) whose inputs are in x (or
+// the given pointers); leaves z, act, out.
+static hipError_t forward(scp3g_ctx *ctx, MlpState *s, const unsigned short *x, int64_t np,
+                          bool keepT, const unsigned short *xT_unused)
+{
+    (void)xT_unused;
+    const unsigned short *in = x;
+    int64_t ldin = s->dpad;
+    for (int l = 0; l < 3; ++l) {
+        const Layer &Ly = s->L[l];
+        dim3 grid((unsigned)(Ly.out / 64), (unsigned)(np / 64));
+        hipLaunchKernelGGL(k_gemm_nt<EPI_FWD>, grid, dim3(256), 0, ctx->stream, in, ldin, Ly.Wb,
+                           (int64_t)Ly.in, Ly.in, s->P + Ly.b_off, s->z[l], s->act[l],
+                           keepT ? s->actT[l] : (unsigned short *)nullptr, (int64_t)s->h,
+                           (int64_t)s->cap, (const float *)nullptr, (float *)nullptr, (int64_t)0);
+        in = s->act[l];
+        ldin = s->h;
+    }
+    return hipGetLastError();
+}
+
+static int mlp_ready(scp3g_ctx *ctx)
+{
+    if (!ctx) return -1;
+    if (!ctx->mlp) return fail(ctx, -2, "no surrogate (scp3g_mlp_init)");
+    return 0;
+}
+
+// inputs of np points from mode-major y
+static hipError_t inputs(scp3g_ctx *ctx, MlpState *s, const double *y, int64_t ldy, int64_t np,
+                         unsigned short *x, unsigned short *xT, int64_t ldxT)
+{
+    hipLaunchKernelGGL(k_mlp_input, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, y,
+                       ldy, s->M, s->dpad, np, x, xT, ldxT);
+    return hipGetLastError();
+}
+
+// f(x) for np points of y (normalised network output) into s->out
+static hipError_t eval_f(scp3g_ctx *ctx, MlpState *s, const double *y, int64_t ldy, int64_t np)
+{
+    const int64_t npad = (np + 63) / 64 * 64;
+    hipError_t e = ensure_batch(s, npad);
+    if (e != hipSuccess) return e;
+    if (npad > np)  // padded inputs are finite (zero rows)
+        hipMemsetAsync(s->x + (size_t)np * s->dpad, 0, (size_t)(npad - np) * s->dpad * 2, ctx->stream);
+    if ((e = inputs(ctx, s, y, ldy, np, s->x, nullptr, 0)) != hipSuccess) return e;
+    if ((e = forward(ctx, s, s->x, npad, false, nullptr)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_head, dim3((unsigned)((npad + 3) / 4)), dim3(256), 0, ctx->stream,
+                       s->act[2], s->z[2], s->h, s->P + s->w3_off, s->P + s->b3_off, npad,
+                       (const float *)nullptr, 0.f, s->out, (float *)nullptr, (float *)nullptr,
+                       (unsigned short *)nullptr, (unsigned short *)nullptr, (int64_t)0);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_weights(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts,
+                              const double *E, int mode, double scale, double *w)
+{
+    MlpState *s = ctx->mlp;
+    if (!s) return hipErrorInvalidValue;
+    hipError_t e = eval_f(ctx, s, y, ldy, npts);
+    if (e != hipSuccess) return e;
+    const int64_t npad = (npts + 255) / 256 * 256;
+    hipLaunchKernelGGL(k_mlp_weights, dim3((unsigned)(npad / 256)), dim3(256), 0, ctx->stream,
+                       s->out, y, ldy, ctx->d_omega, ctx->nmodes0, s->V0 + s->t_mean, s->t_std,
+                       npts, npad, E, mode, scale, w);
+    return hipGetLastError();
+}
+
+// one forward + backward on batch [b0, b0 + B) of the training set; leaves
+// the gradient in s->G and the batch's summed squared error in *lossd.
+static hipError_t grad_step(scp3g_ctx *ctx, MlpState *s, int64_t b0, int64_t B, float *lossd)
+{
+    hipError_t e;
+    const unsigned short *x = s->Xtr + (size_t)b0 * s->dpad;
+    const unsigned short *xT = s->XtrT + b0;  // ld = ntrain
+    if ((e = forward(ctx, s, x, B, true, nullptr)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_head, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, ctx->stream,
+                       s->act[2], s->z[2], s->h, s->P + s->w3_off, s->P + s->b3_off, B,
+                       s->Ttr + b0, 1.0f / (float)B, s->out, s->sq, s->dout, s->dl[2], s->dlT[2],
+                       (int64_t)s->cap);
+    hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, ctx->stream, s->sq, B, lossd);
+    // head: w3, b3
+    hipLaunchKernelGGL(k_head_grad, dim3((unsigned)(s->h + 1)), dim3(256), 0, ctx->stream,
+                       s->actT[2], (int64_t)s->cap, s->dout, B, s->G + s->w3_off, s->G + s->b3_off);
+    for (int l = 2; l >= 0; --l) {
+        const Layer &Ly = s->L[l];
+        // bias gradient: row sums of delta^T
+        hipLaunchKernelGGL(k_rowsum_bf16, dim3((unsigned)Ly.out), dim3(256), 0, ctx->stream,
+                           s->dlT[l], (int64_t)s->cap, B, s->G + Ly.b_off);
+        // weight gradient G[o][i] = sum_p delta[p][o] a_in[p][i]
+        const unsigned short *inT = l > 0 ? s->actT[l - 1] : xT;
+        const int64_t ldinT = l > 0 ? s->cap : s->ntrain;
+        hipLaunchKernelGGL(k_gemm_nt<EPI_WGRAD>, dim3((unsigned)(Ly.in / 64), (unsigned)(Ly.out / 64)),
+                           dim3(256), 0, ctx->stream, s->dlT[l], (int64_t)s->cap, inT, ldinT, (int)B,
+                           (const float *)nullptr, (float *)nullptr, (unsigned short *)nullptr,
+                           (unsigned short *)nullptr, (int64_t)0, (int64_t)0, (const float *)nullptr,
+                           s->G + Ly.w_off, (int64_t)Ly.in);
+        if (l > 0) {
+            // delta_{l-1}[p][i] = sum_o delta_l[p][o] W[o][i] * silu'(z_{l-1}[p][i])
+            hipLaunchKernelGGL(k_gemm_nt<EPI_BWD>, dim3((unsigned)(Ly.in / 64), (unsigned)(B / 64)),
+                               dim3(256), 0, ctx->stream, s->dl[l], (int64_t)s->h, Ly.WbT,
+                               (int64_t)Ly.out, Ly.out, (const float *)nullptr, (float *)nullptr,
+                               s->dl[l - 1], s->dlT[l - 1], (int64_t)s->h, (int64_t)s->cap,
+                               s->z[l - 1], (float *)nullptr, (int64_t)0);
+        }
+    }
+    return hipGetLastError();
+}
+
+static uint64_t splitmix64(uint64_t &x)
+{
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace scp3g
+
+using namespace scp3g;
+
+extern "C" {
+
+int scp3g_mlp_init(scp3g_ctx *ctx, int hidden, uint64_t seed)
+{
+    if (!ctx) return -1;
+    if (ctx->nmodes0 <= 0) return fail(ctx, -2, "system not set (scp3g_set_system)");
+    if (hidden < 64 || hidden % 64) return fail(ctx, -4, "hidden width must be a positive multiple of 64");
+    hipSetDevice(ctx->device);
+    mlp_release(ctx);
+    MlpState *s = new MlpState();
+    ctx->mlp = s;
+    s->M = ctx->nmodes0;
+    s->dpad = (s->M + 63) / 64 * 64;
+    s->h = hidden;
+    int64_t off = 0;
+    const int ins[3] = {s->dpad, hidden, hidden};
+    for (int l = 0; l < 3; ++l) {
+        s->L[l].in = ins[l];
+        s->L[l].out = hidden;
+        s->L[l].w_off = off;
+        off += (int64_t)hidden * ins[l];
+        s->L[l].b_off = off;
+        off += hidden;
+    }
+    s->w3_off = off;
+    off += hidden;
+    s->b3_off = off;
+    off += 1;
+    s->nparam = off;
+    // Glorot-uniform weights from a splitmix64 stream; zero biases; inputs
+    // beyond the M normal coordinates get zero weights
+    std::vector<float> P((size_t)off, 0.0f);
+    uint64_t st = seed;
+    auto uni = [&]() { return (double)(splitmix64(st) >> 11) * 0x1p-53; };
+    for (int l = 0; l < 3; ++l) {
+        const int fan_in = l == 0 ? s->M : hidden;
+        const double a = std::sqrt(6.0 / (fan_in + hidden));
+        for (int o = 0; o < hidden; ++o)
+            for (int i = 0; i < ins[l]; ++i)
+                P[(size_t)s->L[l].w_off + (size_t)o * ins[l] + i] =
+                    (l == 0 && i >= s->M) ? 0.0f : (float)((2.0 * uni() - 1.0) * a);
+    }
+    const double a3 = std::sqrt(6.0 / (hidden + 1));
+    for (int j = 0; j < hidden; ++j) P[(size_t)s->w3_off + j] = (float)((2.0 * uni() - 1.0) * a3);
+    int rc;
+    hipError_t e = hipSuccess;
+    for (float **p : {&s->P, &s->G, &s->m, &s->v})
+        if (e == hipSuccess) e = hipMalloc((void **)p, (size_t)off * 4);
+    for (int l = 0; l < 3; ++l) {
+        if (e == hipSuccess) e = hipMalloc((void **)&s->L[l].Wb, (size_t)hidden * ins[l] * 2);
+        if (e == hipSuccess) e = hipMalloc((void **)&s->L[l].WbT, (size_t)hidden * ins[l] * 2);
+    }
+    if ((rc = check_hip(ctx, e, "alloc mlp"))) return rc;
+    hipMemcpy(s->P, P.data(), P.size() * 4, hipMemcpyHostToDevice);
+    hipMemset(s->G, 0, P.size() * 4);
+    hipMemset(s->m, 0, P.size() * 4);
+    hipMemset(s->v, 0, P.size() * 4);
+    // V0: the potential at the reference geometry, hartree (FP64 CPU)
+    std::vector<double> xa(ctx->x0.size());
+    for (size_t i = 0; i < xa.size(); ++i) xa[i] = kBohr * ctx->x0[i];
+    s->V0 = (ctx->pes == SCP3G_PES_TTM3F ? ttm3f_energy_host(ctx->nw, xa.data(), nullptr)
+                                         : qtip4pf_energy_host(ctx->nw, xa.data())) /
+            kAutokcalpmol;
+    return check_hip(ctx, refresh_bf16(ctx, s), "mlp bf16");
+}
+
+int scp3g_mlp_param_count(const scp3g_ctx *ctx, int64_t *n)
+{
+    if (!ctx || !n) return -1;
+    *n = ctx->mlp ? ctx->mlp->nparam : 0;
+    return 0;
+}
+
+int scp3g_mlp_get_params(scp3g_ctx *ctx, float *P_host, double *norm_host)
+{
+    int rc = mlp_ready(ctx);
+    if (rc) return rc;
+    MlpState *s = ctx->mlp;
+    hipStreamSynchronize(ctx->stream);
+    if (P_host) hipMemcpy(P_host, s->P, (size_t)s->nparam * 4, hipMemcpyDeviceToHost);
+    if (norm_host) {
+        norm_host[0] = s->V0;
+        norm_host[1] = s->t_mean;
+        norm_host[2] = s->t_std;
+    }
+    return 0;
+}
+
+int scp3g_mlp_set_params(scp3g_ctx *ctx, const float *P_host, const double *norm_host)
+{
+    int rc = mlp_ready(ctx);
+    if (rc) return rc;
+    MlpState *s = ctx->mlp;
+    if (P_host) hipMemcpy(s->P, P_host, (size_t)s->nparam * 4, hipMemcpyHostToDevice);
+    if (norm_host) {
+        s->V0 = norm_host[0];
+        s->t_mean = norm_host[1];
+        s->t_std = norm_host[2];
+    }
+    return check_hip(ctx, refresh_bf16(ctx, s), "mlp bf16");
+}
+
+// Load a training set from host arrays: y_host[npts x M] point-major normal
+// coordinates (already divided by sqrt(2)), t_host[npts] normalised targets.
+int scp3g_mlp_load_data(scp3g_ctx *ctx, const double *y_host, const float *t_host, int64_t npts)
+{
+    int rc = mlp_ready(ctx);
+    if (rc) return rc;
+    MlpState *s = ctx->mlp;
+    if (npts <= 0 || npts % 64) return fail(ctx, -4, "training points must be a positive multiple of 64");
+    mfree(s->Xtr);
+    mfree(s->XtrT);
+    mfree(s->Ttr);
+    hipError_t e = hipMalloc((void **)&s->Xtr, (size_t)npts * s->dpad * 2);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->XtrT, (size_t)npts * s->dpad * 2);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->Ttr, (size_t)npts * 4);
+    if ((rc = check_hip(ctx, e, "alloc training set"))) return rc;
+    double *d_y = nullptr;
+    if ((rc = check_hip(ctx, hipMalloc(&d_y, (size_t)npts * s->M * 8), "alloc y"))) return rc;
+    std::vector<double> ym((size_t)npts * s->M);
+    for (int64_t p = 0; p < npts; ++p)
+        for (int k = 0; k < s->M; ++k) ym[(size_t)k * npts + p] = y_host[(size_t)p * s->M + k];
+    hipMemcpy(d_y, ym.data(), ym.size() * 8, hipMemcpyHostToDevice);
+    hipMemcpy(s->Ttr, t_host, (size_t)npts * 4, hipMemcpyHostToDevice);
+    inputs(ctx, s, d_y, npts, npts, s->Xtr, s->XtrT, npts);
+    rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "inputs");
+    hipFree(d_y);
+    s->ntrain = npts;
+    return rc;
+}
+
+// Sample a training set with the sampler itself: Sobol seeds skip..skip+npts-1,
+// the context's potential in FP64, targets normalised to zero mean and unit
+// variance (their mean and scale are kept for evaluation).
+int scp3g_mlp_sample_data(scp3g_ctx *ctx, int64_t skip, int64_t npts)
+{
+    int rc = mlp_ready(ctx);
+    if (rc) return rc;
+    MlpState *s = ctx->mlp;
+    if (npts <= 0 || npts % 64) return fail(ctx, -4, "training points must be a positive multiple of 64");
+    const int M = s->M;
+    std::vector<double> yh((size_t)npts * M), raw(npts);
+    const int64_t T = 4096;
+    double *d_y = nullptr, *d_crd = nullptr, *d_E = nullptr, *d_raw = nullptr;
+    hipError_t e = hipMalloc(&d_y, (size_t)M * T * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_crd, (size_t)9 * ctx->nw * T * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_E, (size_t)T * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_raw, (size_t)T * 8);
+    if ((rc = check_hip(ctx, e, "alloc sampling"))) return rc;
+    std::vector<double> yt((size_t)M * T);
+    for (int64_t done = 0; done < npts && !rc; done += T) {
+        const int64_t np = std::min<int64_t>(T, npts - done);
+        e = launch_sobol(ctx, ctx->d_sobol, M, skip + done, np, 1, d_y, T, nullptr);
+        if (e == hipSuccess) e = launch_map(ctx, d_y, T, np, d_crd, T, 0);
+        if (e == hipSuccess)
+            e = ctx->pes == SCP3G_PES_TTM3F ? launch_ttm3f(ctx, ctx->nw, np, d_crd, T, 0, d_E)
+                                            : launch_qtip4pf(ctx, ctx->nw, np, d_crd, T, 0, d_E);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_mlp_targets, dim3((unsigned)((np + 255) / 256)), dim3(256), 0,
+                               ctx->stream, d_E, d_y, T, ctx->d_omega, M, s->V0, np, d_raw);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(yt.data(), d_y, yt.size() * 8, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(raw.data() + done, d_raw, np * 8, hipMemcpyDeviceToHost);
+        rc = check_hip(ctx, e, "sample training set");
+        for (int64_t p = 0; p < np; ++p)
+            for (int k = 0; k < M; ++k) yh[(size_t)(done + p) * M + k] = yt[(size_t)k * T + p];
+    }
+    hipFree(d_y);
+    hipFree(d_crd);
+    hipFree(d_E);
+    hipFree(d_raw);
+    if (rc) return rc;
+    double mean = 0.0, var = 0.0;
+    for (double r : raw) mean += r;
+    mean /= (double)npts;
+    for (double r : raw) var += (r - mean) * (r - mean);
+    var /= (double)npts;
+    s->t_mean = mean;
+    s->t_std = var > 0 ? std::sqrt(var) : 1.0;
+    std::vector<float> t(npts);
+    for (int64_t p = 0; p < npts; ++p) t[p] = (float)((raw[p] - mean) / s->t_std);
+    return scp3g_mlp_load_data(ctx, yh.data(), t.data(), npts);
+}
+
+// Adam on batches of `batch` consecutive training points (cycling through
+// the set); loss_host (nullable, `steps` entries) gets each step's mean
+// squared error in normalised units.
+int scp3g_mlp_train(scp3g_ctx *ctx, int steps, int batch, double lr, float *loss_host)
+{
+    int rc = mlp_ready(ctx);
+    if (rc) return rc;
+    MlpState *s = ctx->mlp;
+    if (!s->Xtr) return fail(ctx, -2, "no training set");
+    if (batch <= 0 || batch % 64 || batch > s->ntrain) return fail(ctx, -4, "bad batch size");
+    if ((rc = check_hip(ctx, ensure_batch(s, batch), "alloc batch"))) return rc;
+    const int64_t nb = s->ntrain / batch;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    hipEvent_t ev;
+    timer_begin(ctx, "mlp_train", &ev);
+    std::vector<float> losses;
+    for (int it = 0; it < steps; ++it) {
+        const int64_t b0 = (it % nb) * batch;
+        float *ld = s->lossd + (it % (64 * 1024));
+        hipError_t e = grad_step(ctx, s, b0, batch, ld);
+        ++s->step;
+        const float c1 = 1.f - std::pow(b1, (float)s->step), c2 = 1.f - std::pow(b2, (float)s->step);
+        hipLaunchKernelGGL(k_adam, dim3((unsigned)((s->nparam + 255) / 256)), dim3(256), 0, ctx->stream,
+                           s->P, s->G, s->m, s->v, s->nparam, (float)lr, b1, b2, eps, c1, c2);
+        if (e == hipSuccess) e = refresh_bf16(ctx, s);
+        if ((rc = check_hip(ctx, e, "train step"))) return rc;
+        if (loss_host && ((it + 1) % (64 * 1024) == 0 || it + 1 == steps)) {
+            const int64_t first = (it / (64 * 1024)) * (64 * 1024);
+            const int64_t cnt = it + 1 - first;
+            std::vector<float> tmp(cnt);
+            hipMemcpyAsync(tmp.data(), s->lossd, cnt * 4, hipMemcpyDeviceToHost, ctx->stream);
+            hipStreamSynchronize(ctx->stream);
+            for (int64_t j = 0; j < cnt; ++j) loss_host[first + j] = tmp[j] / (float)batch;
+        }
+    }
+    timer_end(ctx, "mlp_train", ev);
+    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "train");
+}
+
+// Gradient of the batch loss at the current parameters (no update) over the
+// first `batch` training points; grad_host[nparam], *loss = mean squared error.
+int scp3g_mlp_grad(scp3g_ctx *ctx, int batch, float *grad_host, float *loss)
+{
+    int rc = mlp_ready(ctx);
+    if (rc) return rc;
+    MlpState *s = ctx->mlp;
+    if (!s->Xtr || batch <= 0 || batch % 64 || batch > s->ntrain) return fail(ctx, -4, "bad batch");
+    if ((rc = check_hip(ctx, ensure_batch(s, batch), "alloc batch"))) return rc;
+    if ((rc = check_hip(ctx, grad_step(ctx, s, 0, batch, s->lossd), "grad"))) return rc;
+    hipStreamSynchronize(ctx->stream);
+    if (grad_host) hipMemcpy(grad_host, s->G, (size_t)s->nparam * 4, hipMemcpyDeviceToHost);
+    if (loss) {
+        float l;
+        hipMemcpy(&l, s->lossd, 4, hipMemcpyDeviceToHost);
+        *loss = l / (float)batch;
+    }
+    return 0;
+}
+
+// V_nn (hartree) of npts points y_dev (mode-major, ld ldy, halved normals).
+int scp3g_mlp_eval(scp3g_ctx *ctx, const double *y_dev, int64_t ldy, int64_t npts, double *V_dev)
+{
+    int rc = mlp_ready(ctx);
+    if (rc) return rc;
+    MlpState *s = ctx->mlp;
+    if (npts <= 0 || ldy < npts || !y_dev || !V_dev) return fail(ctx, -4, "bad arguments");
+    const int64_t T = 1 << 16;
+    for (int64_t done = 0; done < npts && !rc; done += T) {
+        const int64_t np = std::min<int64_t>(T, npts - done);
+        hipError_t e = eval_f(ctx, s, y_dev + done, ldy, np);
+        if (e == hipSuccess) {
+            // V_nn = w_nn + 1/2 sum omega y^2: mode 0 gives w_nn
+            hipLaunchKernelGGL(k_mlp_weights, dim3((unsigned)((np + 255) / 256)), dim3(256), 0,
+                               ctx->stream, s->out, y_dev + done, ldy, ctx->d_omega, s->M,
+                               s->V0 + s->t_mean, s->t_std, np, np, (const double *)nullptr, 0, 1.0,
+                               V_dev + done);
+            e = hipGetLastError();
+        }
+        rc = check_hip(ctx, e, "mlp eval");
+    }
+    return rc;
+}
+
+// Control-variate estimator of H: the surrogate weights over Sobol seeds
+// skip_nn .. skip_nn+P_nn-1, each scaled 1/P_nn, plus the correction
+// bV - w_nn over seeds skip_t .. skip_t+P_t-1, scaled 1/P_t.  The running
+// sum then normalises to 1 (scp3g_h_download divides by it).
+int scp3g_accumulate_cv(scp3g_ctx *ctx, int64_t skip_nn, int64_t P_nn, int64_t skip_t, int64_t P_t)
+{
+    int rc = mlp_ready(ctx);
+    if (rc) return rc;
+    if (P_nn <= 0 || P_t <= 0 || skip_nn < 0 || skip_t < 0) return fail(ctx, -4, "bad point counts");
+    for (int pass = 0; pass < 2; ++pass) {
+        const int64_t skip = pass ? skip_t : skip_nn, P = pass ? P_t : P_nn;
+        if ((rc = scp3g_accumulate_mode(ctx, skip, P, pass ? 2 : 1, 1.0 / (double)P))) return rc;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -81,12 +81,22 @@ struct scp3g_ctx {
     size_t part_bytes = 0;
     int64_t h_n = 0;
     int64_t points_done = 0;
+    // normaliser of the running sum: the number of points for the plain
+    // estimator, the sum of per-point scales for the control-variate one
+    double norm_sum = 0.0;
+    // MLP surrogate state (mlp.hip), owned by the context
+    struct MlpState *mlp = nullptr;
 
     // timing
     bool timing = false;
     std::map<std::string, scp3g::KernelTimer> timers;
     std::vector<hipEvent_t> event_pool;
 };
+
+// Sobol seeds skip..skip+npts-1 through the tile pipeline with weight mode
+// wmode (0 plain, 1 surrogate, 2 control-variate correction) and scale.
+extern "C" int scp3g_accumulate_mode(scp3g_ctx *ctx, int64_t skip, int64_t npts, int wmode,
+                                     double scale);
 
 namespace scp3g {
 
@@ -112,6 +122,17 @@ hipError_t launch_syrk(scp3g_ctx *ctx, int mode, int64_t n, const void *B, int64
                        const double *w, int64_t npts, double *H, double *Hc);
 hipError_t launch_finalize(scp3g_ctx *ctx, const double *H, int64_t n, double scale,
                            const double *diag_add, double *out);
+
+hipError_t launch_mlp_weights(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts,
+                              const double *E, int mode, double scale, double *w);
+int run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale);
+hipError_t launch_qtip4pf(scp3g_ctx *ctx, int nw, int64_t npts, const void *crd, int64_t ldc,
+                          int fp32, double *E);
+hipError_t launch_ttm3f(scp3g_ctx *ctx, int nw, int64_t npts, const void *crd, int64_t ldc,
+                        int fp32, double *E);
+double qtip4pf_energy_host(int nw, const double *crd);
+double ttm3f_energy_host(int nw, const double *crd, int *iters);
+void mlp_release(scp3g_ctx *ctx);
 
 // host helpers
 std::vector<double> ho_diagonal(const scp3g_ctx *ctx);  // HO diagonal + unselected ZPE
