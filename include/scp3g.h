@@ -86,6 +86,9 @@ int scp3g_sub_basis_size(const scp3g_ctx *ctx, int level, int64_t *n);
 int scp3g_set_pes(scp3g_ctx *ctx, const char *name);
 /* SCP3G_FP64 or SCP3G_MIXED */
 int scp3g_set_precision(scp3g_ctx *ctx, int mode);
+/* mixed form SYRK kernel: 1 = signed 256 x 256-tile kernel on X = sqrt|w| B
+ * (default), 0 = 64 x 64-tile kernel on B and w */
+int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast);
 /* points per device tile (multiple of 256; default 4096) */
 int scp3g_set_tile(scp3g_ctx *ctx, int tile);
 

@@ -42,6 +42,7 @@ _SIGS = {
     "scp3g_set_pes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "scp3g_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "scp3g_set_mixed_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_reset": (ctypes.c_int, [ctypes.c_void_p]),
     "scp3g_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "scp3g_accumulate_normals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
@@ -184,6 +185,9 @@ class Context:
 
     def set_tile(self, tile):
         self.check(self.L.scp3g_set_tile(self._h, tile))
+
+    def set_mixed_kernel(self, fast):
+        self.check(self.L.scp3g_set_mixed_kernel(self._h, 1 if fast else 0))
 
     # whole path ---------------------------------------------------------
     def reset(self):

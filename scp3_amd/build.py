@@ -22,6 +22,7 @@ SOURCES = [
     "basis.hip",
     "syrk.hip",
     "syrk_mixed.hip",
+    "syrk_fast.hip",
     "eigen.hip",
     "mlp.hip",
     "context.cpp",

@@ -190,10 +190,12 @@ static int ensure_work(scp3g_ctx *ctx)
         return rc;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_E, sizeof(double) * T), "alloc E"))) return rc;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_w, sizeof(double) * T), "alloc w"))) return rc;
-    if ((rc = check_hip(ctx, hipMalloc(&ctx->d_B, sizeof(double) * rows * T), "alloc B"))) return rc;
-    if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_B, 0, sizeof(double) * rows * T, ctx->stream),
-                        "clear B")))
-        return rc;
+    // FP64 table (rows x T doubles) or, in the mixed form, the two BF16
+    // planes of X (rows padded to 256) and the sign words, whichever is larger
+    const size_t rows_f = (size_t)fast_rows_pad(n);
+    const size_t bbytes = std::max(sizeof(double) * rows * T, 4 * rows_f * T + T / 8 + 256);
+    if ((rc = check_hip(ctx, hipMalloc(&ctx->d_B, bbytes), "alloc B"))) return rc;
+    if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_B, 0, bbytes, ctx->stream), "clear B"))) return rc;
     ctx->work_tile = ctx->tile;
     ctx->work_n = n;
     return 0;
@@ -270,6 +272,7 @@ void scp3g_destroy(scp3g_ctx *ctx)
     dfree(ctx->d_desc);
     release_tiles(ctx);
     mlp_release(ctx);
+    release_fast_tiles(ctx);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -403,6 +406,13 @@ int scp3g_set_precision(scp3g_ctx *ctx, int mode)
     return 0;
 }
 
+int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast)
+{
+    if (!ctx) return -1;
+    ctx->mixed_fast = fast != 0;
+    return 0;
+}
+
 int scp3g_set_tile(scp3g_ctx *ctx, int tile)
 {
     if (!ctx) return -1;
@@ -468,16 +478,33 @@ int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
     }
     if (rc) return rc;
 basis:
-    timer_begin(ctx, "basis", &ev);
-    rc = check_hip(ctx, launch_basis(ctx, ctx->d_y, T, np, ctx->d_B, T, fp32), "basis");
-    timer_end(ctx, "basis", ev);
-    if (rc) return rc;
-    timer_begin(ctx, "syrk", &ev);
-    rc = check_hip(ctx, launch_syrk(ctx, ctx->precision, ctx->nbasis, ctx->d_B, T, ctx->d_w, np,
-                                    ctx->d_H, ctx->d_Hc),
-                   "syrk");
-    timer_end(ctx, "syrk", ev);
-    if (rc) return rc;
+    if (fp32 && ctx->mixed_fast) {
+        // signed split-BF16 operand X = sqrt|w| B in the k-blocked layout
+        const int64_t K = (np + 63) / 64 * 64;
+        const int64_t rows_pad = fast_rows_pad(ctx->nbasis);
+        unsigned short *Xh = reinterpret_cast<unsigned short *>(ctx->d_B);
+        unsigned short *Xl = Xh + (size_t)rows_pad * T;
+        unsigned *sgn = reinterpret_cast<unsigned *>(Xl + (size_t)rows_pad * T);
+        timer_begin(ctx, "basis", &ev);
+        rc = check_hip(ctx, launch_basis_split(ctx, ctx->d_y, T, np, K, ctx->d_w, Xh, Xl, sgn), "basis");
+        timer_end(ctx, "basis", ev);
+        if (rc) return rc;
+        timer_begin(ctx, "syrk", &ev);
+        rc = check_hip(ctx, launch_syrk_x3(ctx, ctx->nbasis, Xh, Xl, sgn, K, ctx->d_H, ctx->d_Hc), "syrk");
+        timer_end(ctx, "syrk", ev);
+        if (rc) return rc;
+    } else {
+        timer_begin(ctx, "basis", &ev);
+        rc = check_hip(ctx, launch_basis(ctx, ctx->d_y, T, np, ctx->d_B, T, fp32), "basis");
+        timer_end(ctx, "basis", ev);
+        if (rc) return rc;
+        timer_begin(ctx, "syrk", &ev);
+        rc = check_hip(ctx, launch_syrk(ctx, ctx->precision, ctx->nbasis, ctx->d_B, T, ctx->d_w, np,
+                                        ctx->d_H, ctx->d_Hc),
+                       "syrk");
+        timer_end(ctx, "syrk", ev);
+        if (rc) return rc;
+    }
     ctx->points_done += np;
     if (wmode == 0) ctx->norm_sum += (double)np;
     else if (wmode == 1) ctx->norm_sum += (double)np * scale;

@@ -58,6 +58,9 @@ struct scp3g_ctx {
     int pes = SCP3G_PES_QTIP4PF;
     int precision = SCP3G_FP64;
     int tile = 4096;
+    // mixed form: the 256 x 256-tile signed SYRK (syrk_fast.hip) instead of
+    // the 64 x 64-tile kernel (syrk_mixed.hip)
+    bool mixed_fast = true;
 
     // device copies of the set-up
     uint64_t *d_sobol = nullptr;  // nmodes0 x 62
@@ -130,6 +133,13 @@ hipError_t launch_qtip4pf(scp3g_ctx *ctx, int nw, int64_t npts, const void *crd,
                           int fp32, double *E);
 hipError_t launch_ttm3f(scp3g_ctx *ctx, int nw, int64_t npts, const void *crd, int64_t ldc,
                         int fp32, double *E);
+int64_t fast_rows_pad(int64_t n);
+hipError_t launch_basis_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts, int64_t K,
+                              const double *w, unsigned short *Xh, unsigned short *Xl, unsigned *sgn);
+hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
+                          const unsigned short *Xl, const unsigned *sgn, int64_t K, double *H,
+                          double *Hc);
+void release_fast_tiles(scp3g_ctx *ctx);
 double qtip4pf_energy_host(int nw, const double *crd);
 double ttm3f_energy_host(int nw, const double *crd, int *iters);
 void mlp_release(scp3g_ctx *ctx);

@@ -1,0 +1,344 @@
+// Large-n mixed-precision SYRK: H += B diag(w) B^T as a signed symmetric
+// rank-K update of X = sqrt|w| B on split-BF16 MFMA.
+//
+// Replaces dsyr2k (ScaledMatrixElements.cpp:942-949,970) for basis sizes
+// where the upper triangle has thousands of 256 x 256 tiles (n >= ~4000).
+//
+// With s_p = sign(w_p) and x_p = sqrt|w_p| b_p,  B diag(w) B^T = sum_p s_p
+// x_p x_p^T, so one operand array serves both sides of the product and the
+// sign is applied to the left operand as it leaves LDS (an XOR of the BF16
+// sign bits).  x is FP32 and is split exactly into hi = bf16(x),
+// lo = bf16(x - hi); the product is hi.hi + hi.lo + lo.hi (relative error
+// ~2^-16), accumulated in FP32 by v_mfma_f32_16x16x32_bf16.
+//
+// Layout (written by k_basis_split): Xh, Xl are [K/32][rows_pad][32] BF16,
+// i.e. each 32-point chunk of every row is 64 contiguous bytes and the
+// chunks of consecutive rows are contiguous, so a workgroup's 256 rows x 32
+// points are one 16 KiB contiguous block per plane.  sgn[K/32] holds the
+// sign bits of each 32-point chunk.
+//
+// Tiling: a workgroup (8 waves, 2 x 4) owns a 256 x 256 tile (ti <= tj);
+// each wave owns 128 x 64 = 8 x 4 MFMA blocks (128 FP32 accumulators per
+// lane), two waves per SIMD.  Per 32-point step the workgroup stages A-hi, A-lo, B-hi, B-lo
+// (64 KiB) in LDS, double buffered (128 KiB).  LDS rows are 64 bytes (four
+// 16-byte chunks); chunk c of row r is stored at position c ^ ((r >> 2) & 3)
+// so that the 16 lanes of an MFMA operand read (rows r..r+15, one chunk)
+// hit 16 distinct 4-bank groups.  The epilogue folds each tile's FP32 sum
+// into the FP64 running H with Kahan compensation (Hc), or writes split-K
+// partials.
+#include "scp3g_internal.h"
+
+namespace scp3g {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kFT = 256;  // workgroup tile
+
+__device__ __forceinline__ unsigned short bf16_rne_u(float f)
+{
+    const unsigned u = __float_as_uint(f);
+    const unsigned bias = 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)((u + bias) >> 16);
+}
+
+// x = sqrt|w| * bra, split into hi/lo planes in the k-blocked layout, and
+// the sign words.  One thread per (point, row group) as in k_basis.
+__global__ void __launch_bounds__(256)
+    k_basis_split(const double *__restrict__ y, int64_t ldy, const int *__restrict__ modes, int nsel,
+                  const RowDesc *__restrict__ desc, int64_t n, int64_t rows_pad, int rows_per_block,
+                  int64_t npts, int64_t K, const double *__restrict__ w,
+                  unsigned short *__restrict__ Xh, unsigned short *__restrict__ Xl,
+                  unsigned *__restrict__ sgn)
+{
+    extern __shared__ float hsf[];  // [3][nsel][64]
+    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    const bool live = i < npts;
+    for (int m = grp; m < nsel; m += 4) {
+        const float q = live ? (float)y[(size_t)modes[m] * ldy + i] : 0.f;
+        hsf[(0 * nsel + m) * 64 + lane] = 1.41421356f * q;
+        hsf[(1 * nsel + m) * 64 + lane] = (2.f * q * q - 1.f) * 0.70710678f;
+        hsf[(2 * nsel + m) * 64 + lane] = (2.f * q * q - 3.f) * q * 0.57735027f;
+    }
+    const double wi = live ? w[i] : 0.0;
+    if (blockIdx.y == 0 && grp == 0) {
+        const unsigned long long neg = __ballot(wi < 0.0);
+        if (lane == 0) sgn[i >> 5] = (unsigned)neg;
+        if (lane == 32) sgn[i >> 5] = (unsigned)(neg >> 32);
+    }
+    __syncthreads();
+    const float sw = (float)sqrt(fabs(wi));
+    const int64_t a0 = (int64_t)blockIdx.y * rows_per_block;
+    int64_t a1 = a0 + rows_per_block;
+    if (a1 > rows_pad) a1 = rows_pad;
+    if (i >= K) return;
+    const size_t chunk = (size_t)(i >> 5) * rows_pad * 32 + (i & 31);
+    for (int64_t a = a0 + grp; a < a1; a += 4) {
+        float v = 0.f;
+        if (a < n && live) {
+            const RowDesc d = desc[a];
+            float f[3];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const int mode = d.f[t] & 0xffff, deg = d.f[t] >> 16;
+                f[t] = deg ? hsf[((deg - 1) * nsel + mode) * 64 + lane] : 1.f;
+            }
+            v = sw * ((f[0] * f[1]) * f[2]);
+        }
+        const unsigned short h = bf16_rne_u(v);
+        const unsigned short l = bf16_rne_u(v - __uint_as_float((unsigned)h << 16));
+        Xh[chunk + (size_t)a * 32] = h;
+        Xl[chunk + (size_t)a * 32] = l;
+    }
+}
+
+__device__ __forceinline__ void kahan_add2(double &h, double &c, double v)
+{
+    const double yv = v - c;
+    const double t = h + yv;
+    c = (t - h) - yv;
+    h = t;
+}
+
+__global__ void __launch_bounds__(512, 1)
+    k_syrk_x3(const uint4 *__restrict__ Xh, const uint4 *__restrict__ Xl,
+              const unsigned *__restrict__ sgn, int64_t rows_pad, const int2 *__restrict__ tiles,
+              int ntiles, int kb_per_split, int nkb, int64_t n, double *__restrict__ H,
+              double *__restrict__ Hc, float *__restrict__ part)
+{
+    // [stage][plane Ah, Al, Bh, Bl][row * 4 + swizzled chunk]
+    __shared__ uint4 lds[2][4][kFT * 4];
+    const int2 t = tiles[blockIdx.x];
+    const int split = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 2, wc = wave & 3;
+    const int li = lane & 15, lg = lane >> 4;
+    const int kb0 = split * kb_per_split;
+    int kb1 = kb0 + kb_per_split;
+    if (kb1 > nkb) kb1 = nkb;
+    const size_t rowA = (size_t)t.x * kFT, rowB = (size_t)t.y * kFT;
+    const size_t planeStride = (size_t)rows_pad * 4;  // uint4 per 32-point chunk plane
+
+    f4 acc[8][4];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+    // each thread moves 2 chunks of each plane per step (chunk q = tid + 512 j,
+    // q = row * 4 + chunk); the A planes are staged while the first half of
+    // the column blocks is computed and the B planes during the second half
+    uint4 st[4];
+    auto gload = [&](int kb, size_t row0) {
+        const size_t base = (size_t)kb * planeStride + row0 * 4;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int q = tid + 512 * j;
+            st[j] = Xh[base + q];
+            st[2 + j] = Xl[base + q];
+        }
+    };
+    auto lstore = [&](int sg, int plane0) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int q = tid + 512 * j;
+            const int r = q >> 2, c = q & 3;
+            const int p = r * 4 + (c ^ ((r >> 2) & 3));
+            lds[sg][plane0][p] = st[j];
+            lds[sg][plane0 + 1][p] = st[2 + j];
+        }
+    };
+    if (kb0 < kb1) {
+        gload(kb0, rowA);
+        lstore(0, 0);
+        gload(kb0, rowB);
+        lstore(0, 2);
+    }
+    __syncthreads();
+    int s = 0;
+    for (int kb = kb0; kb < kb1; ++kb) {
+        const bool more = kb + 1 < kb1;
+        // sign flips for this lane's 8 points (8 lg .. 8 lg + 7)
+        const unsigned bits = (sgn[kb] >> (8 * lg)) & 0xffu;
+        uint4 fl;
+        fl.x = ((bits & 1u) << 15) | (((bits >> 1) & 1u) << 31);
+        fl.y = (((bits >> 2) & 1u) << 15) | (((bits >> 3) & 1u) << 31);
+        fl.z = (((bits >> 4) & 1u) << 15) | (((bits >> 5) & 1u) << 31);
+        fl.w = (((bits >> 6) & 1u) << 15) | (((bits >> 7) & 1u) << 31);
+        bf16x8 ah[8], al[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const int r = wr * 128 + 16 * a + li;
+            const int p = r * 4 + (lg ^ ((r >> 2) & 3));
+            uint4 vh = lds[s][0][p], vl = lds[s][1][p];
+            vh.x ^= fl.x; vh.y ^= fl.y; vh.z ^= fl.z; vh.w ^= fl.w;
+            vl.x ^= fl.x; vl.y ^= fl.y; vl.z ^= fl.z; vl.w ^= fl.w;
+            ah[a] = __builtin_bit_cast(bf16x8, vh);
+            al[a] = __builtin_bit_cast(bf16x8, vl);
+        }
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            if (more) gload(kb + 1, half ? rowB : rowA);
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) {
+                const int b = 2 * half + bb;
+                const int r = wc * 64 + 16 * b + li;
+                const int p = r * 4 + (lg ^ ((r >> 2) & 3));
+                const bf16x8 bh = __builtin_bit_cast(bf16x8, lds[s][2][p]);
+                const bf16x8 bl = __builtin_bit_cast(bf16x8, lds[s][3][p]);
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
+                }
+            }
+            if (more) lstore(s ^ 1, half ? 2 : 0);
+        }
+        __syncthreads();
+        s ^= 1;
+    }
+
+    // D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4
+    const int oj = lane & 15, oi = 4 * (lane >> 4);
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int tr0 = wr * 128 + 16 * a + oi;
+            const int tc = wc * 64 + 16 * b + oj;
+            const int64_t col = (int64_t)rowB + tc;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int tr = tr0 + e;
+                const int64_t row = (int64_t)rowA + tr;
+                const float v = acc[a][b][e];
+                if (part) {
+                    part[((size_t)split * ntiles + blockIdx.x) * (kFT * kFT) + (size_t)tr * kFT + tc] = v;
+                } else if (row <= col && col < n) {
+                    const size_t idx = (size_t)col * n + row;
+                    double h = H[idx], c = Hc[idx];
+                    kahan_add2(h, c, (double)v);
+                    H[idx] = h;
+                    Hc[idx] = c;
+                }
+            }
+        }
+}
+
+__global__ void __launch_bounds__(256)
+    k_syrk_x3_fold(const float *__restrict__ part, const int2 *__restrict__ tiles, int ntiles,
+                   int nsplit, int64_t n, double *__restrict__ H, double *__restrict__ Hc)
+{
+    const int tile = blockIdx.x;
+    const int2 t = tiles[tile];
+    for (int e = blockIdx.y * 256 + threadIdx.x; e < kFT * kFT; e += 256 * gridDim.y) {
+        const int tr = e / kFT, tc = e % kFT;
+        const int64_t row = (int64_t)t.x * kFT + tr, col = (int64_t)t.y * kFT + tc;
+        if (row > col || col >= n) continue;
+        double s = 0.0;
+        for (int sp = 0; sp < nsplit; ++sp) s += (double)part[((size_t)sp * ntiles + tile) * (kFT * kFT) + e];
+        const size_t idx = (size_t)col * n + row;
+        double h = H[idx], c = Hc[idx];
+        kahan_add2(h, c, s);
+        H[idx] = h;
+        Hc[idx] = c;
+    }
+}
+
+// --------------------------------------------------------------------------
+
+struct FastTiles {
+    int64_t n = -1;
+    int2 *d = nullptr;
+    int ntiles = 0;
+};
+static std::map<scp3g_ctx *, FastTiles> g_ftiles;
+
+static hipError_t fast_tiles(scp3g_ctx *ctx, int64_t n, const int2 **tiles, int *ntiles)
+{
+    FastTiles &ft = g_ftiles[ctx];
+    if (ft.n != n) {
+        if (ft.d) hipFree(ft.d);
+        ft.d = nullptr;
+        const int T = (int)((n + kFT - 1) / kFT);
+        std::vector<int2> v;
+        for (int j = 0; j < T; ++j)
+            for (int i = 0; i <= j; ++i) v.push_back(make_int2(i, j));
+        hipError_t e = hipMalloc(&ft.d, v.size() * sizeof(int2));
+        if (e != hipSuccess) return e;
+        e = hipMemcpy(ft.d, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return e;
+        ft.n = n;
+        ft.ntiles = (int)v.size();
+    }
+    *tiles = ft.d;
+    *ntiles = ft.ntiles;
+    return hipSuccess;
+}
+
+void release_fast_tiles(scp3g_ctx *ctx)
+{
+    auto it = g_ftiles.find(ctx);
+    if (it != g_ftiles.end()) {
+        if (it->second.d) hipFree(it->second.d);
+        g_ftiles.erase(it);
+    }
+}
+
+int64_t fast_rows_pad(int64_t n) { return (n + kFT - 1) / kFT * kFT; }
+
+hipError_t launch_basis_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts, int64_t K,
+                              const double *w, unsigned short *Xh, unsigned short *Xl, unsigned *sgn)
+{
+    const int64_t n = ctx->nbasis, rows_pad = fast_rows_pad(n);
+    const int nsel = (int)ctx->modes.size();
+    const size_t smem = (size_t)3 * nsel * 64 * sizeof(float);
+    if (smem > 64 * 1024) return hipErrorInvalidValue;
+    const int64_t pblocks = K / 64;
+    int rpb = 256;
+    while (rpb > 16 && pblocks * ((rows_pad + rpb - 1) / rpb) < 2048) rpb /= 2;
+    dim3 grid((unsigned)pblocks, (unsigned)((rows_pad + rpb - 1) / rpb));
+    hipLaunchKernelGGL(k_basis_split, grid, dim3(256), smem, ctx->stream, y, ldy, ctx->d_modes, nsel,
+                       ctx->d_desc, n, rows_pad, rpb, npts, K, w, Xh, Xl, sgn);
+    return hipGetLastError();
+}
+
+hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
+                          const unsigned short *Xl, const unsigned *sgn, int64_t K, double *H,
+                          double *Hc)
+{
+    const int2 *tiles;
+    int ntiles;
+    hipError_t e = fast_tiles(ctx, n, &tiles, &ntiles);
+    if (e != hipSuccess) return e;
+    const int nkb = (int)(K / 32);
+    int nsplit = 1;
+    while (ntiles * nsplit < 512 && nkb / (nsplit * 2) >= 8) nsplit *= 2;
+    const int kbs = (nkb + nsplit - 1) / nsplit;
+    nsplit = (nkb + kbs - 1) / kbs;
+    float *part = nullptr;
+    if (nsplit > 1) {
+        const size_t need = (size_t)nsplit * ntiles * kFT * kFT * sizeof(float);
+        if (ctx->part_bytes < need) {
+            if (ctx->d_part) hipFree(ctx->d_part);
+            ctx->d_part = nullptr;
+            ctx->part_bytes = 0;
+            if ((e = hipMalloc(&ctx->d_part, need)) != hipSuccess) return e;
+            ctx->part_bytes = need;
+        }
+        part = reinterpret_cast<float *>(ctx->d_part);
+    }
+    const int64_t rows_pad = fast_rows_pad(n);
+    hipLaunchKernelGGL(k_syrk_x3, dim3((unsigned)ntiles, (unsigned)nsplit), dim3(512), 0, ctx->stream,
+                       reinterpret_cast<const uint4 *>(Xh), reinterpret_cast<const uint4 *>(Xl), sgn,
+                       rows_pad, tiles, ntiles, kbs, nkb, n, H, Hc, part);
+    e = hipGetLastError();
+    if (e != hipSuccess || !part) return e;
+    hipLaunchKernelGGL(k_syrk_x3_fold, dim3((unsigned)ntiles, 16), dim3(256), 0, ctx->stream, part, tiles,
+                       ntiles, nsplit, n, H, Hc);
+    return hipGetLastError();
+}
+
+}  // namespace scp3g
