@@ -23,6 +23,7 @@ SOURCES = [
     "syrk.hip",
     "syrk_mixed.hip",
     "syrk_fast.hip",
+    "syrk64_fast.hip",
     "eigen.hip",
     "mlp.hip",
     "context.cpp",

@@ -182,7 +182,7 @@ static int ensure_work(scp3g_ctx *ctx)
     if (ctx->work_tile == ctx->tile && ctx->work_n == n && ctx->d_B) return 0;
     free_work(ctx);
     const size_t T = (size_t)ctx->tile;
-    const size_t rows = (size_t)((n + 63) / 64 * 64);
+    const size_t rows = (size_t)rows_pad128(n);  // FP64 table rows (zero padded)
     int rc;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_y, sizeof(double) * ctx->nmodes0 * T), "alloc y")))
         return rc;
@@ -273,6 +273,7 @@ void scp3g_destroy(scp3g_ctx *ctx)
     release_tiles(ctx);
     mlp_release(ctx);
     release_fast_tiles(ctx);
+    release_tiles128(ctx);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -693,8 +694,8 @@ int scp3g_h_accumulate(scp3g_ctx *ctx, int mode, int64_t n, const void *B_dev, i
     if (mode != SCP3G_FP64 && mode != SCP3G_MIXED) return fail(ctx, -4, "bad mode");
     if (n <= 0 || npts < 0 || !B_dev || !w_dev || !H_dev) return fail(ctx, -4, "bad arguments");
     if (mode == SCP3G_MIXED && !Hc_dev) return fail(ctx, -4, "mixed mode needs Hc");
-    // the kernels read 64-row tiles and 16/32-point chunks: the caller's B must
-    // hold rows up to round_up(n, 64) and points up to round_up(npts, 32),
+    // the kernels read 128-row tiles and 16/32-point chunks: the caller's B must
+    // hold rows up to round_up(n, 128) and points up to round_up(npts, 32),
     // zero beyond n / npts, and w zero beyond npts
     if (ldb % 32 || ldb < (npts + 31) / 32 * 32) return fail(ctx, -4, "ldb must be a multiple of 32 covering npts");
     hipSetDevice(ctx->device);

@@ -58,8 +58,9 @@ struct scp3g_ctx {
     int pes = SCP3G_PES_QTIP4PF;
     int precision = SCP3G_FP64;
     int tile = 4096;
-    // mixed form: the 256 x 256-tile signed SYRK (syrk_fast.hip) instead of
-    // the 64 x 64-tile kernel (syrk_mixed.hip)
+    // LDS-tiled SYRK kernels (syrk_fast.hip: signed 256 x 256 split-BF16;
+    // syrk64_fast.hip: 128 x 128 FP64) instead of the 64 x 64 direct-load
+    // kernels (syrk.hip, syrk_mixed.hip)
     bool mixed_fast = true;
 
     // device copies of the set-up
@@ -140,6 +141,10 @@ hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
                           const unsigned short *Xl, const unsigned *sgn, int64_t K, double *H,
                           double *Hc);
 void release_fast_tiles(scp3g_ctx *ctx);
+int64_t rows_pad128(int64_t n);
+hipError_t launch_syrk_f64_lds(scp3g_ctx *ctx, int64_t n, const double *B, int64_t ldb,
+                               const double *w, int64_t npts, double *H);
+void release_tiles128(scp3g_ctx *ctx);
 double qtip4pf_energy_host(int nw, const double *crd);
 double ttm3f_energy_host(int nw, const double *crd, int *iters);
 void mlp_release(scp3g_ctx *ctx);

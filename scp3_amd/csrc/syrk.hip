@@ -176,6 +176,7 @@ hipError_t launch_syrk(scp3g_ctx *ctx, int mode, int64_t n, const void *B, int64
     if (npts <= 0) return hipSuccess;
     if (mode == SCP3G_MIXED)
         return launch_syrk_mixed(ctx, n, (const float *)B, ldb, w, npts, H, Hc);
+    if (ctx->mixed_fast) return launch_syrk_f64_lds(ctx, n, (const double *)B, ldb, w, npts, H);
     const int2 *tiles;
     int ntiles;
     hipError_t e = get_tiles(ctx, n, &tiles, &ntiles);

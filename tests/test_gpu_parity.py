@@ -146,7 +146,7 @@ def test_basis_bit_exact(ctx, hexamer, M2, M3, npts):
 def _syrk_case(ctx, n, P, mode):
     rng = np.random.default_rng(n + P)
     ld = (P + 255) // 256 * 256
-    rows = (n + 63) // 64 * 64
+    rows = (n + 127) // 128 * 128
     B = np.zeros((rows, ld))
     B[:n, :P] = rng.normal(size=(n, P))
     w = np.zeros(ld)
