@@ -1,18 +1,23 @@
-"""Benchmark: Sobol points/s into H and time to the lowest 10 levels.
+"""Benchmark: Sobol points/s into H and time to the lowest levels.
 
-Workload (default): water hexamer relaxed under q-TIP4P/F (6 H2O, 48 modes),
-basis of total quanta <= 2 (1225 states), 2^20 Sobol points (skip 2^30) in
-4096-point tiles.  One step = one full pass of the hot path over the 2^20
-points (sample -> map -> potential -> basis -> SYRK into the FP64 running H)
-followed by the lowest 10 levels on the GPU.  Both GPU forms are run: the
-plain FP64 form and the mixed form (FP32 potential and basis, 3-term
-split-BF16 MFMA SYRK folded into FP64 H).  `value` is the mixed form's
-points/s into H; the FP64 form, time-to-levels of both and the level
-difference are reported beside it.  The CPU baseline is the FP64 oracle path
-(Sobol/BSM, map, serial potential loop, get_bra, scipy/OpenBLAS dsyr2k) on a
-bounded prefix of the same points.
+Default workload (preset c2): water hexamer relaxed under q-TIP4P/F (6 H2O,
+48 modes), basis of total quanta <= 2 (1225 states), 2^20 Sobol points
+(skip 2^30) in 4096-point tiles.  One step = one full pass of the hot path
+over the 2^20 points (sample -> map -> potential -> basis -> SYRK into the
+FP64 running H) followed by the lowest 10 levels on the GPU.  Both GPU forms
+run: the plain FP64 form and the mixed form (FP32 potential and basis,
+3-term split-BF16 MFMA SYRK folded into FP64 H).  `value` is the mixed
+form's points/s into H; the FP64 form, the time-to-levels of both and the
+level difference are reported beside it.  The CPU baseline is the FP64
+oracle path (Sobol/BSM, map, serial potential loop, get_bra, OpenBLAS
+dsyr2k) on a bounded prefix of the same points.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--points LOG2]
+Other presets: c1 (dimer, 91 states, 2^16 points), c3 (hexamer under
+TTM3-F, 20825 states, 2^21 points, 16384-point tiles), c4 (c3 with the MLP
+control variate: 2^24 surrogate + 2^19 true points against plain FP64 at
+2^21).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--preset c2]
 """
 import argparse
 import json
@@ -27,7 +32,14 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
-PEAK_HBM_GBS = 8000.0
+
+PRESETS = {
+    "c1": dict(config="dimer_qtip4pf", n2=12, n3=0, points=16, tile=4096, levels=5, forms="fp64,mixed"),
+    "c2": dict(config="hexamer_qtip4pf", n2=48, n3=0, points=20, tile=4096, levels=10, forms="fp64,mixed"),
+    "c3": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=16384, levels=10, forms="fp64,mixed"),
+    "c4": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=16384, levels=10, forms="fp64,cv",
+               cv_nn=24, cv_t=19, mlp_hidden=256, mlp_steps=20000, mlp_batch=4096, mlp_pairs=18),
+}
 
 
 def parse():
@@ -35,38 +47,58 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--points", type=int, default=20, help="log2 of Sobol points per step")
-    ap.add_argument("--config", default="hexamer_qtip4pf")
-    ap.add_argument("--nmodes2", type=int, default=48)
-    ap.add_argument("--nmodes3", type=int, default=0)
-    ap.add_argument("--levels", type=int, default=10)
+    ap.add_argument("--preset", default="c2", choices=sorted(PRESETS))
+    ap.add_argument("--points", type=int, default=None, help="log2 of Sobol points per step")
+    ap.add_argument("--forms", default=None)
     ap.add_argument("--cpu-sample", type=int, default=13, help="log2 points of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--cv-terms", type=int, default=1)
     return ap.parse_args()
 
 
-def run_form(c, sysd, mode, P, skip, steps, warmup, k):
+def setup(c, sysd, pr, mode):
     import scp3_amd
     c.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
     c.set_basis(sysd["modes"], sysd["nmodes2"], sysd["nmodes3"])
-    c.set_pes("qtip4pf" if "qtip4pf" in sysd["name"] else "ttm3f")
+    c.set_pes("ttm3f" if "ttm3f" in pr["config"] else "qtip4pf")
     c.set_precision(mode)
-    c.set_tile(4096)
-    for _ in range(warmup):
+    c.set_tile(pr["tile"])
+
+
+def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
+    import scp3_amd
+    mode = scp3_amd.FP64 if form == "fp64" else scp3_amd.MIXED
+    setup(c, sysd, pr, mode)
+    train_s = 0.0
+    if form == "cv":
+        t0 = time.perf_counter()
+        c.mlp_init(pr["mlp_hidden"], seed=0)
+        c.mlp_sample_data(1 << 40, 1 << pr["mlp_pairs"])
+        loss = c.mlp_train(pr["mlp_steps"], pr["mlp_batch"], 1e-3)
+        c.sync()
+        train_s = time.perf_counter() - t0
+        extra["mlp_final_loss"] = float(loss[-100:].mean())
+        extra["mlp_train_s"] = train_s
+
+    def one():
         c.reset()
-        c.accumulate(skip, P)
+        if form == "cv":
+            # correction points: the first 2^cv_t of the surrogate's Sobol sequence
+            c.accumulate_cv(skip, 1 << pr["cv_nn"], skip, 1 << pr["cv_t"])
+        else:
+            c.accumulate(skip, P)
+
+    for _ in range(warmup):
+        one()
         c.lowest_levels(k)
     c.sync()
     c.set_timing(True)
-    t_acc = 0.0
-    t_lev = 0.0
+    t_acc = t_lev = 0.0
     ev = None
     for _ in range(steps):
         c.sync()
         t0 = time.perf_counter()
-        c.reset()
-        c.accumulate(skip, P)
+        one()
         c.sync()
         t1 = time.perf_counter()
         ev = c.lowest_levels(k)
@@ -75,12 +107,12 @@ def run_form(c, sysd, mode, P, skip, steps, warmup, k):
         t_acc += t1 - t0
         t_lev += t2 - t1
     kt = {}
-    for name in ("sobol", "map", "pes", "weight", "basis", "syrk", "symv", "symfill"):
-        ms, cnt = c.kernel_time(name)
-        kt[name] = (ms, cnt)
+    for name in ("sobol", "map", "pes", "weight", "mlp", "basis", "syrk", "symv", "symfill"):
+        kt[name] = c.kernel_time(name)
     c.set_timing(False)
-    return dict(t_acc=t_acc / steps, t_lev=t_lev / steps, levels=ev, kernels=kt,
-                n=c.basis_size())
+    npts = c.points_done()
+    return dict(t_acc=t_acc / steps, t_lev=t_lev / steps, levels=ev, kernels=kt, n=c.basis_size(),
+                points=npts, train_s=train_s)
 
 
 def cpu_baseline(sysd, skip, logS):
@@ -94,6 +126,7 @@ def cpu_baseline(sysd, skip, logS):
     modes = sysd["modes"]
     n = oracle.basis_size(modes.size, sysd["nmodes2"], sysd["nmodes3"])
     cores = len(os.sched_getaffinity(0))
+    pes = oracle.ttm3f if "ttm3f" in sysd["name"] else oracle.qtip4pf
     t = {}
     t0 = time.perf_counter()
     y = oracle.sobol_normals(M0, skip, S)
@@ -102,7 +135,7 @@ def cpu_baseline(sysd, skip, logS):
     r = BOHR * (sysd["x0"][None, :] + y @ sysd["MUa"].T)
     t["map"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    V = np.array([oracle.qtip4pf(r[i]) for i in range(S)]) / AUTOKCALPMOL
+    V = np.array([pes(r[i]) for i in range(S)]) / AUTOKCALPMOL
     t["pes"] = time.perf_counter() - t0
     bV = V - 0.5 * (y * sysd["omega"] * y).sum(1)
     t0 = time.perf_counter()
@@ -116,46 +149,51 @@ def cpu_baseline(sysd, skip, logS):
     t["syrk"] = time.perf_counter() - t0
     total = sum(t.values())
     return dict(value=S / total, unit="points/s", cores=cores, kind="port",
-                sample="%d Sobol points (skip 2^30) of the same workload; phases %s s; "
-                       "FP64 oracle path: C Sobol/BSM, numpy map, serial potential loop, "
-                       "get_bra, OpenBLAS dsyr2k (threads = %d)"
-                       % (S, {k: round(v, 4) for k, v in t.items()}, cores),
-                phase_s=t)
+                sample="%d Sobol points (skip 2^30) of the same workload; phases %s s; FP64 oracle "
+                       "path: C Sobol/BSM, numpy map, serial potential loop, get_bra, OpenBLAS "
+                       "dsyr2k (%d threads)" % (S, {k: round(v, 4) for k, v in t.items()}, cores))
 
 
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md)" % a.gpus, file=sys.stderr)
+              "(DESIGN.md describes how the path would shard)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     import scp3_amd
     from scp3_amd import system
 
-    sysd = system.prepare(system.input_path(a.config), nmodes2=a.nmodes2, nmodes3=a.nmodes3)
-    sysd["name"] = a.config
-    P = 1 << a.points
+    pr = dict(PRESETS[a.preset])
+    if a.points is not None:
+        pr["points"] = a.points
+    if a.forms:
+        pr["forms"] = a.forms
+    sysd = system.prepare(system.input_path(pr["config"]), nmodes2=pr["n2"], nmodes3=pr["n3"])
+    sysd["name"] = pr["config"]
+    P = 1 << pr["points"]
     skip = 1 << 30
-    forms = a.forms.split(",")
-    res = {}
+    forms = pr["forms"].split(",")
+    res, extra = {}, {}
     with scp3_amd.Context(0) as c:
+        c.L.scp3g_set_cv_terms(c.handle, a.cv_terms)
         for f in forms:
-            mode = scp3_amd.FP64 if f == "fp64" else scp3_amd.MIXED
-            print("bench: form %s, %d points, n=%s" % (f, P, "?"), file=sys.stderr, flush=True)
-            res[f] = run_form(c, sysd, mode, P, skip, a.steps, a.warmup, a.levels)
+            print("bench: preset %s form %s" % (a.preset, f), file=sys.stderr, flush=True)
+            res[f] = run_form(c, sysd, pr, f, P, skip, a.steps, a.warmup, pr["levels"], extra)
+            print("bench: form %s accumulate %.4f s, levels %.4f s" % (f, res[f]["t_acc"], res[f]["t_lev"]),
+                  file=sys.stderr, flush=True)
     n = res[forms[0]]["n"]
-    main_form = "mixed" if "mixed" in res else forms[0]
+    main_form = "mixed" if "mixed" in res else ("cv" if "cv" in res else forms[0])
     r = res[main_form]
     # dominant kernel: the SYRK; algorithmic flops per launch = n(n+1) * tile points
     ms, cnt = r["kernels"]["syrk"]
     avg_s = ms / 1e3 / max(cnt, 1)
-    tile = 4096
-    flops = float(n) * (n + 1) * tile
-    peak = PEAK_BF16_TFLOPS if main_form == "mixed" else PEAK_FP64_TFLOPS
+    tile = pr["tile"]
+    flops = float(n) * (n + 1) * min(tile, P)
+    peak = PEAK_FP64_TFLOPS if main_form == "fp64" else PEAK_BF16_TFLOPS
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
     out = {
         "metric": "Sobol pts/s into H + time to lowest-10 levels at 0.5 cm^-1, GPU forms vs plain",
-        "value": P / r["t_acc"],
+        "value": r["points"] / r["t_acc"],
         "unit": "points/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -164,34 +202,36 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3/fp32 SYRK + fp64 H" if main_form == "mixed" else "f64",
-        "data": "synthetic: seed-built, relaxed cluster; Sobol points",
-        "config": {"workload": "%s n=%d, 2^%d Sobol points, 4096-point tiles, lowest %d levels"
-                               % (a.config, n, a.points, a.levels),
+        "dtype": "f64 H; bf16x3 MFMA SYRK, f32 potential/basis" if main_form != "fp64" else "f64",
+        "data": "synthetic: seed-built relaxed cluster (scp3_amd/data), Sobol points",
+        "config": {"workload": "%s: %s, %d states, 2^%d Sobol points, %d-point tiles, lowest %d levels"
+                               % (a.preset, pr["config"], n, pr["points"], tile, pr["levels"]),
                    "basis_states": n, "points": P, "form": main_form},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "kernel": "syrk (%s)" % main_form, "avg_launch_ms": avg_s * 1e3,
+                     "kernel": "syrk (%s form)" % main_form, "avg_launch_ms": avg_s * 1e3,
                      "flops_per_launch": flops},
         "forms": {},
     }
     for f, rr in res.items():
         out["forms"][f] = {
-            "points_per_s": P / rr["t_acc"],
-            "time_to_levels_s": rr["t_acc"] + rr["t_lev"],
+            "points": rr["points"],
+            "points_per_s": rr["points"] / rr["t_acc"],
+            "time_to_levels_s": rr["t_acc"] + rr["t_lev"] + rr["train_s"],
+            "accumulate_s": rr["t_acc"],
             "eigensolve_s": rr["t_lev"],
             "levels_cm1": [round(float(v) * scp3_amd.AUTOCM, 4) for v in rr["levels"]],
-            "kernel_ms_per_step": {k: round(v[0] / a.steps, 4) for k, v in rr["kernels"].items() if v[1]},
+            "kernel_ms_per_step": {kk: round(v[0] / a.steps, 4) for kk, v in rr["kernels"].items() if v[1]},
         }
-    if "fp64" in res and "mixed" in res:
-        d = np.abs(res["mixed"]["levels"] - res["fp64"]["levels"]).max() * scp3_amd.AUTOCM
-        out["mixed_vs_fp64_max_level_diff_cm1"] = float(d)
-        out["speedup_time_to_levels_mixed_vs_fp64"] = (
-            (res["fp64"]["t_acc"] + res["fp64"]["t_lev"]) / (r["t_acc"] + r["t_lev"]))
+        if f == "cv":
+            out["forms"][f].update(extra)
+    if "fp64" in res and main_form != "fp64":
+        d = np.abs(res[main_form]["levels"] - res["fp64"]["levels"]).max() * scp3_amd.AUTOCM
+        out["max_level_diff_vs_fp64_cm1"] = float(d)
+        t64 = out["forms"]["fp64"]["time_to_levels_s"]
+        out["speedup_time_to_levels_vs_fp64"] = t64 / out["forms"][main_form]["time_to_levels_s"]
     if not a.no_cpu:
-        cb = cpu_baseline(sysd, skip, a.cpu_sample)
-        cb.pop("phase_s")
-        out["cpu_baseline"] = cb
+        out["cpu_baseline"] = cpu_baseline(sysd, skip, a.cpu_sample)
     print(json.dumps(out))
 
 

@@ -178,6 +178,9 @@ int scp3g_mlp_eval(scp3g_ctx *ctx, const double *y_dev, int64_t ldy, int64_t npt
  */
 int scp3g_accumulate_cv(scp3g_ctx *ctx, int64_t skip_nn, int64_t P_nn, int64_t skip_t,
                         int64_t P_t);
+/* mixed form: split-BF16 terms used for the surrogate term of the control
+ * variate, 3 (default, hi.hi + hi.lo + lo.hi) or 1 (hi.hi) */
+int scp3g_set_cv_terms(scp3g_ctx *ctx, int terms);
 
 /* ---- per-stage entry points (device pointers; used by parity tests) ---- */
 

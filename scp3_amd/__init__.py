@@ -43,6 +43,7 @@ _SIGS = {
     "scp3g_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_mixed_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "scp3g_set_cv_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_reset": (ctypes.c_int, [ctypes.c_void_p]),
     "scp3g_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "scp3g_accumulate_normals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),

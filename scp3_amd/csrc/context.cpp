@@ -414,6 +414,14 @@ int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast)
     return 0;
 }
 
+int scp3g_set_cv_terms(scp3g_ctx *ctx, int terms)
+{
+    if (!ctx) return -1;
+    if (terms != 1 && terms != 3) return fail(ctx, -4, "terms must be 1 or 3");
+    ctx->cv_one_term = terms == 1;
+    return 0;
+}
+
 int scp3g_set_tile(scp3g_ctx *ctx, int tile)
 {
     if (!ctx) return -1;
@@ -491,7 +499,9 @@ basis:
         timer_end(ctx, "basis", ev);
         if (rc) return rc;
         timer_begin(ctx, "syrk", &ev);
-        rc = check_hip(ctx, launch_syrk_x3(ctx, ctx->nbasis, Xh, Xl, sgn, K, ctx->d_H, ctx->d_Hc), "syrk");
+        const int terms = (wmode == 1 && ctx->cv_one_term) ? 1 : 3;
+        rc = check_hip(ctx, launch_syrk_x3(ctx, ctx->nbasis, Xh, Xl, sgn, K, ctx->d_H, ctx->d_Hc, terms),
+                       "syrk");
         timer_end(ctx, "syrk", ev);
         if (rc) return rc;
     } else {

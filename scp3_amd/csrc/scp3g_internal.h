@@ -62,6 +62,8 @@ struct scp3g_ctx {
     // syrk64_fast.hip: 128 x 128 FP64) instead of the 64 x 64 direct-load
     // kernels (syrk.hip, syrk_mixed.hip)
     bool mixed_fast = true;
+    // control variate, mixed form: the surrogate term with hi.hi products only
+    bool cv_one_term = false;
 
     // device copies of the set-up
     uint64_t *d_sobol = nullptr;  // nmodes0 x 62
@@ -139,7 +141,7 @@ hipError_t launch_basis_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int6
                               const double *w, unsigned short *Xh, unsigned short *Xl, unsigned *sgn);
 hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
                           const unsigned short *Xl, const unsigned *sgn, int64_t K, double *H,
-                          double *Hc);
+                          double *Hc, int terms);
 void release_fast_tiles(scp3g_ctx *ctx);
 int64_t rows_pad128(int64_t n);
 hipError_t launch_syrk_f64_lds(scp3g_ctx *ctx, int64_t n, const double *B, int64_t ldb,

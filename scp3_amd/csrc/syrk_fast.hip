@@ -101,6 +101,9 @@ __device__ __forceinline__ void kahan_add2(double &h, double &c, double v)
     h = t;
 }
 
+// TERMS = 3: hi.hi + hi.lo + lo.hi; TERMS = 1: hi.hi only (relative error
+// ~2^-8 per product; used for the surrogate term of the control variate)
+template <int TERMS>
 __global__ void __launch_bounds__(512, 1)
     k_syrk_x3(const uint4 *__restrict__ Xh, const uint4 *__restrict__ Xl,
               const unsigned *__restrict__ sgn, int64_t rows_pad, const int2 *__restrict__ tiles,
@@ -190,8 +193,10 @@ __global__ void __launch_bounds__(512, 1)
 #pragma unroll
                 for (int a = 0; a < 8; ++a) {
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
+                    if (TERMS == 3) {
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
+                    }
                 }
             }
             if (more) lstore(s ^ 1, half ? 2 : 0);
@@ -307,7 +312,7 @@ hipError_t launch_basis_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int6
 
 hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
                           const unsigned short *Xl, const unsigned *sgn, int64_t K, double *H,
-                          double *Hc)
+                          double *Hc, int terms)
 {
     const int2 *tiles;
     int ntiles;
@@ -331,9 +336,16 @@ hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
         part = reinterpret_cast<float *>(ctx->d_part);
     }
     const int64_t rows_pad = fast_rows_pad(n);
-    hipLaunchKernelGGL(k_syrk_x3, dim3((unsigned)ntiles, (unsigned)nsplit), dim3(512), 0, ctx->stream,
-                       reinterpret_cast<const uint4 *>(Xh), reinterpret_cast<const uint4 *>(Xl), sgn,
-                       rows_pad, tiles, ntiles, kbs, nkb, n, H, Hc, part);
+    if (terms == 1)
+        hipLaunchKernelGGL(k_syrk_x3<1>, dim3((unsigned)ntiles, (unsigned)nsplit), dim3(512), 0,
+                           ctx->stream, reinterpret_cast<const uint4 *>(Xh),
+                           reinterpret_cast<const uint4 *>(Xl), sgn, rows_pad, tiles, ntiles, kbs, nkb, n,
+                           H, Hc, part);
+    else
+        hipLaunchKernelGGL(k_syrk_x3<3>, dim3((unsigned)ntiles, (unsigned)nsplit), dim3(512), 0,
+                           ctx->stream, reinterpret_cast<const uint4 *>(Xh),
+                           reinterpret_cast<const uint4 *>(Xl), sgn, rows_pad, tiles, ntiles, kbs, nkb, n,
+                           H, Hc, part);
     e = hipGetLastError();
     if (e != hipSuccess || !part) return e;
     hipLaunchKernelGGL(k_syrk_x3_fold, dim3((unsigned)ntiles, 16), dim3(256), 0, ctx->stream, part, tiles,

@@ -1,0 +1,73 @@
+"""The SCP3 and eigensolver drivers (scp3_amd/bin): reference options,
+output files, sM_*.h5 round trip."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "scp3_amd", "bin")
+DIMER = os.path.join(ROOT, "scp3_amd", "data", "dimer_qtip4pf.dat")
+
+
+def run(args, cwd):
+    return subprocess.run([os.path.join(BIN, args[0])] + args[1:], cwd=cwd, capture_output=True,
+                          text=True, timeout=300)
+
+
+def test_without_G_says_so(tmp_path):
+    r = run(["SCP3", "-N", "10", DIMER], str(tmp_path))
+    assert r.returncode != 0 and "-G" in r.stderr
+
+
+def test_unknown_potential(tmp_path):
+    r = run(["SCP3", "-G", "-p", "nonsense", DIMER], str(tmp_path))
+    assert r.returncode != 0 and "Unknown H2O potential" in r.stderr
+    r = run(["SCP3", "-G", "-p", "whbb", DIMER], str(tmp_path))
+    assert r.returncode != 0 and "WHBB" in r.stderr
+
+
+def test_missing_input(tmp_path):
+    r = run(["SCP3", "-G"], str(tmp_path))
+    assert r.returncode != 0 and "No input file" in r.stderr
+    r = run(["SCP3", "-G", str(tmp_path / "absent.dat")], str(tmp_path))
+    assert r.returncode != 0 and "File not found" in r.stderr
+
+
+@pytest.mark.gpu
+def test_scp3_run_and_eigensolver(tmp_path):
+    import scp3_amd
+    from scp3_amd import system
+    print("cli case: dimer -N 17", flush=True)
+    r = run(["SCP3", "-G", "-N", "17", "-2", "12", "--levels", "5", DIMER], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert len(open(tmp_path / "omega0.dat").read().split()) == 18
+    e0 = np.loadtxt(tmp_path / "E0_sd.dat")
+    assert e0.shape == (8, 4) and list(e0[:, 0]) == [16384 * (k + 1) for k in range(8)]
+    np.testing.assert_allclose(e0[:, 3], e0[:, 2] - e0[:, 1], atol=1e-8)
+    assert os.path.exists(tmp_path / "E0_t.dat")
+    assert os.path.exists(tmp_path / "sM_0131072.h5")
+    lv = np.loadtxt(tmp_path / "levels.dat")
+    # the same run through the Python binding
+    sysd = system.prepare(DIMER, nmodes2=12, nmodes3=0)
+    with scp3_amd.Context(0) as c:
+        c.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
+        c.set_basis(sysd["modes"], 12, 0)
+        c.set_pes("qtip4pf")
+        c.reset()
+        c.accumulate(1 << 30, 1 << 17)
+        ev = c.lowest_levels(5)
+        e1 = c.lowest_levels_sub(c.sub_basis_size(1), 1)[0]
+    np.testing.assert_allclose(lv[:, 0], ev, rtol=0, atol=1e-12)
+    assert abs(e0[-1, 1] - e1 * scp3_amd.AUTOCM) < 1e-6
+    # eigensolver on the FP32 sM file (eigensolver.cpp:83-109 output format)
+    r = run(["eigensolver", "sM_0131072.h5", "5"], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    txt = open(tmp_path / "eigenpairs_0131072.dat").read().split("\n")
+    assert txt[0] == "# Lowest 5 eigenvalues"
+    evs = np.array([float(v) for v in txt[1:6]])
+    assert txt[6] == "# Corresponding eigenvectors, one per line."
+    np.testing.assert_allclose(evs, ev, rtol=0, atol=2e-7)  # FP32 storage of H
+    vec = np.array([float(v) for v in txt[7].split()])
+    assert vec.size == 91 and abs(np.linalg.norm(vec) - 1) < 1e-6

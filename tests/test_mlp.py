@@ -81,3 +81,66 @@ def test_training_lowers_the_loss(gpu_ctx_factory, hexamer):
     print("mlp loss: first %.4f  last %.5f" % (loss[:10].mean(), loss[-50:].mean()))
     assert np.all(np.isfinite(loss))
     assert loss[-50:].mean() < 0.2 * loss[:10].mean()
+
+
+def _small_basis(ctx, hexamer, mode):
+    import scp3_amd
+    ctx.set_system(hexamer["x0"], hexamer["omega"], hexamer["MUa"])
+    ctx.set_basis(np.arange(48, dtype=np.int32), 12, 0)   # 127 states
+    ctx.set_pes("qtip4pf")
+    ctx.set_precision(mode)
+    ctx.set_tile(4096)
+
+
+@pytest.mark.gpu
+def test_control_variate_identity(gpu_ctx_factory, hexamer):
+    """With the same points in both terms the control-variate estimator is
+    the plain estimator: sum w_nn + sum (w - w_nn) = sum w."""
+    import scp3_amd
+    ctx = gpu_ctx_factory(0)
+    _small_basis(ctx, hexamer, scp3_amd.FP64)
+    ctx.mlp_init(64, seed=3)
+    ctx.mlp_sample_data(1 << 30, 4096)
+    ctx.mlp_train(50, 1024, 1e-3)
+    P = 3 * 4096 + 100
+    ctx.reset()
+    ctx.accumulate(1 << 30, P)
+    H = ctx.h_download()
+    ctx.reset()
+    ctx.accumulate_cv(1 << 30, P, 1 << 30, P)
+    Hcv = ctx.h_download()
+    assert np.abs(Hcv - H).max() < 1e-12 * np.abs(H).max()
+
+
+@pytest.mark.gpu
+def test_control_variate_beats_plain_qmc(gpu_ctx_factory, hexamer):
+    """At the same number of true-potential points, the trained surrogate as
+    control variate gives a smaller error of H and of the levels than plain
+    QMC (reference: plain FP64 QMC at 2^20 points)."""
+    import scp3_amd
+    ctx = gpu_ctx_factory(0)
+    _small_basis(ctx, hexamer, scp3_amd.FP64)
+    ctx.reset()
+    ctx.accumulate(1 << 30, 1 << 20)
+    Href = ctx.h_download()
+    ev_ref = ctx.lowest_levels(10)
+    Pt = 1 << 12
+    ctx.reset()
+    ctx.accumulate(1 << 31, Pt)          # plain: Pt true points (another stretch of the sequence)
+    Hpl = ctx.h_download()
+    ev_pl = ctx.lowest_levels(10)
+    ctx.mlp_init(256, seed=0)
+    ctx.mlp_sample_data(1 << 40, 1 << 15)
+    loss = ctx.mlp_train(3000, 1024, 1e-3)
+    ctx.reset()
+    ctx.accumulate_cv(1 << 31, 1 << 18, 1 << 31, Pt)
+    Hcv = ctx.h_download()
+    ev_cv = ctx.lowest_levels(10)
+    e_pl = np.abs(Hpl - Href).max()
+    e_cv = np.abs(Hcv - Href).max()
+    l_pl = np.abs(ev_pl - ev_ref).max() * scp3_amd.AUTOCM
+    l_cv = np.abs(ev_cv - ev_ref).max() * scp3_amd.AUTOCM
+    print("control variate: loss %.4f, max|dH| plain %.3e cv %.3e, max|dlevel| plain %.3f cv %.3f cm^-1"
+          % (loss[-100:].mean(), e_pl, e_cv, l_pl, l_cv))
+    assert e_cv < 0.5 * e_pl
+    assert l_cv < l_pl
