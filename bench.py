@@ -50,7 +50,8 @@ def parse():
     ap.add_argument("--preset", default="c2", choices=sorted(PRESETS))
     ap.add_argument("--points", type=int, default=None, help="log2 of Sobol points per step")
     ap.add_argument("--forms", default=None)
-    ap.add_argument("--cpu-sample", type=int, default=13, help="log2 points of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline time budget")
+    ap.add_argument("--cpu-chunk", type=int, default=8192, help="points per CPU baseline chunk")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cv-terms", type=int, default=1)
     return ap.parse_args()
@@ -115,43 +116,46 @@ def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
                 points=npts, train_s=train_s)
 
 
-def cpu_baseline(sysd, skip, logS):
-    """FP64 oracle path on the host: bounded prefix of the same points."""
+def cpu_baseline(sysd, skip, chunk, target_s, max_points):
+    """FP64 oracle path on the host over a bounded prefix of the same points:
+    chunks of `chunk` points until `target_s` seconds or `max_points`."""
     import scipy.linalg.blas as blas
 
     from oracle import oracle
     from scp3_amd import AUTOKCALPMOL, BOHR
-    S = 1 << logS
     M0 = sysd["omega"].size
     modes = sysd["modes"]
     n = oracle.basis_size(modes.size, sysd["nmodes2"], sysd["nmodes3"])
     cores = len(os.sched_getaffinity(0))
     pes = oracle.ttm3f if "ttm3f" in sysd["name"] else oracle.qtip4pf
-    t = {}
-    t0 = time.perf_counter()
-    y = oracle.sobol_normals(M0, skip, S)
-    t["sample"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    r = BOHR * (sysd["x0"][None, :] + y @ sysd["MUa"].T)
-    t["map"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    V = np.array([pes(r[i]) for i in range(S)]) / AUTOKCALPMOL
-    t["pes"] = time.perf_counter() - t0
-    bV = V - 0.5 * (y * sysd["omega"] * y).sum(1)
-    t0 = time.perf_counter()
-    bra = oracle.get_bra(y[:, modes], modes.size, sysd["nmodes2"], sysd["nmodes3"])
-    t["basis"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
+    t = dict(sample=0.0, map=0.0, pes=0.0, basis=0.0, syrk=0.0)
     Mm = np.zeros((n, n), order="F")
-    A = np.asfortranarray(bra.T)
-    Bk = np.asfortranarray(bra.T * bV[None, :])
-    Mm = blas.dsyr2k(0.5, A, Bk, beta=1.0, c=Mm, trans=0, lower=0, overwrite_c=1)
-    t["syrk"] = time.perf_counter() - t0
+    done = 0
+    while done < max_points and sum(t.values()) < target_s:
+        S = chunk
+        t0 = time.perf_counter()
+        y = oracle.sobol_normals(M0, skip + done, S)
+        t1 = time.perf_counter()
+        r = BOHR * (sysd["x0"][None, :] + y @ sysd["MUa"].T)
+        t2 = time.perf_counter()
+        V = np.array([pes(r[i]) for i in range(S)]) / AUTOKCALPMOL
+        bV = V - 0.5 * (y * sysd["omega"] * y).sum(1)
+        t3 = time.perf_counter()
+        bra = oracle.get_bra(y[:, modes], modes.size, sysd["nmodes2"], sysd["nmodes3"])
+        t4 = time.perf_counter()
+        A = np.asfortranarray(bra.T)
+        Bk = np.asfortranarray(bra.T * bV[None, :])
+        Mm = blas.dsyr2k(0.5, A, Bk, beta=1.0, c=Mm, trans=0, lower=0, overwrite_c=1)
+        t5 = time.perf_counter()
+        for k, dt in zip(t, (t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4)):
+            t[k] += dt
+        done += S
     total = sum(t.values())
-    return dict(value=S / total, unit="points/s", cores=cores, kind="port",
-                sample="%d Sobol points (skip 2^30) of the same workload; phases %s s; FP64 oracle "
-                       "path: C Sobol/BSM, numpy map, serial potential loop, get_bra, OpenBLAS "
-                       "dsyr2k (%d threads)" % (S, {k: round(v, 4) for k, v in t.items()}, cores))
+    return dict(value=done / total, unit="points/s", cores=cores, kind="port",
+                sample="%d Sobol points (skip 2^30, %d-point chunks, %.1f s) of the same workload; "
+                       "phases %s s; FP64 oracle path: C Sobol/BSM, numpy map, serial potential loop, "
+                       "get_bra, OpenBLAS dsyr2k (%d threads)"
+                       % (done, chunk, total, {k: round(v, 3) for k, v in t.items()}, cores))
 
 
 def main():
@@ -231,7 +235,7 @@ def main():
         t64 = out["forms"]["fp64"]["time_to_levels_s"]
         out["speedup_time_to_levels_vs_fp64"] = t64 / out["forms"][main_form]["time_to_levels_s"]
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sysd, skip, a.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(sysd, skip, a.cpu_chunk, a.cpu_seconds, P)
     print(json.dumps(out))
 
 

@@ -57,12 +57,14 @@ __host__ __device__ __forceinline__ T monomer(const T *__restrict__ s, int strid
 }
 
 // Energy of one configuration whose coordinate c sits at me[c * st]; mm is
-// scratch for the 3nw M-site coordinates at the same stride.
+// scratch for the 3nw M-site coordinates at the same stride.  Terms are
+// evaluated in T and summed in double: the Coulomb terms are O(100) kcal/mol
+// and cancel to a few kcal/mol, so a float running sum would lose ~1e-4.
 template <typename T>
-__host__ __device__ T qtip4pf_energy(const T *__restrict__ me, T *__restrict__ mm, int st, int nw)
+__host__ __device__ double qtip4pf_energy(const T *__restrict__ me, T *__restrict__ mm, int st, int nw)
 {
     using namespace qtip4pf;
-    T Eint = 0;
+    double Eint = 0;
     for (int w = 0; w < nw; ++w) {
         const T *O = me + 9 * w * st;
 #pragma unroll
@@ -71,7 +73,7 @@ __host__ __device__ T qtip4pf_energy(const T *__restrict__ me, T *__restrict__ m
                 T(gammaM) * O[k * st] + T(gamma1) * (O[(3 + k) * st] + O[(6 + k) * st]);
         Eint += monomer<T>(O, st);
     }
-    T Evdw = 0, Eelec = 0;
+    double Evdw = 0, Eelec = 0;
     for (int a = 0; a < nw; ++a) {
         // H(b) - M(a), qtip4pf.cpp:179-198
         for (int b = 0; b < nw; ++b) {
@@ -137,8 +139,8 @@ __global__ void __launch_bounds__(64)
     const int64_t ic = i < npts ? i : npts - 1;
     for (int c = 0; c < 9 * nw; ++c) s[c * 64 + lane] = crd[(size_t)c * ldc + ic];
     // no barrier needed: every lane only reads its own column
-    const T e = qtip4pf_energy<T>(s + lane, ms + lane, 64, nw);
-    if (i < npts) E[i] = (double)e;
+    const double e = qtip4pf_energy<T>(s + lane, ms + lane, 64, nw);
+    if (i < npts) E[i] = e;
 }
 
 double qtip4pf_energy_host(int nw, const double *crd)

@@ -59,9 +59,10 @@ __global__ void __launch_bounds__(256)
         }
     }
 
-    // D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4
+    // v_mfma_f64_16x16x4f64 result: D[i][j] sits in lane j + 16*(i%4),
+    // element i/4 (rows interleave across the four lane groups)
     const int oj = lane & 15;
-    const int oi = 4 * (lane >> 4);
+    const int oi = lane >> 4;
     const d4 *accs[4] = {&acc00, &acc01, &acc10, &acc11};
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
@@ -70,7 +71,7 @@ __global__ void __launch_bounds__(256)
         const int64_t col = c0 + 16 * bj + oj;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int64_t row = r0 + 16 * bi + oi + e;
+            const int64_t row = r0 + 16 * bi + 4 * e + oi;
             if (part) {
                 const int tr = (int)(row - (int64_t)t.x * kTile);
                 const int tc = (int)(col - (int64_t)t.y * kTile);

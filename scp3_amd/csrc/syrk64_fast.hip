@@ -96,8 +96,9 @@ __global__ void __launch_bounds__(256, 2)
         s ^= 1;
     }
 
-    // D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4
-    const int oj = lane & 15, oi = 4 * (lane >> 4);
+    // v_mfma_f64_16x16x4f64 result: D[i][j] sits in lane j + 16*(i%4),
+    // element i/4 (rows interleave across the four lane groups)
+    const int oj = lane & 15, oi = lane >> 4;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -106,7 +107,7 @@ __global__ void __launch_bounds__(256, 2)
             const int64_t col = rB + tc;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int tr = wr * 64 + 16 * a + oi + e;
+                const int tr = wr * 64 + 16 * a + 4 * e + oi;
                 const int64_t row = rA + tr;
                 if (part)
                     part[((size_t)split * ntiles + blockIdx.x) * (kT64 * kT64) + (size_t)tr * kT64 + tc] =

@@ -42,39 +42,49 @@ __device__ __forceinline__ unsigned short bf16_rne_u(float f)
     return (unsigned short)((u + bias) >> 16);
 }
 
-// x = sqrt|w| * bra, split into hi/lo planes in the k-blocked layout, and
-// the sign words.  One thread per (point, row group) as in k_basis.
+// Sign words: bit j of sgn[c] is set when the weight of point 32 c + j is
+// negative (points beyond npts count as positive, their x is zero).
+__global__ void __launch_bounds__(256)
+    k_sign_words(const double *__restrict__ w, int64_t npts, int64_t K, unsigned *__restrict__ sgn)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool neg = i < npts && w[i] < 0.0;
+    const unsigned long long m = __ballot(neg);
+    const int lane = threadIdx.x & 63;
+    if (i < K) {
+        if (lane == 0) sgn[i >> 5] = (unsigned)m;
+        if (lane == 32) sgn[i >> 5] = (unsigned)(m >> 32);
+    }
+}
+
+// x = sqrt|w| * bra in FP32, split into hi/lo BF16 planes in the k-blocked
+// layout.  A block holds PB points (64, or fewer when many modes are
+// selected) and 256/PB row groups, as in k_basis.
 __global__ void __launch_bounds__(256)
     k_basis_split(const double *__restrict__ y, int64_t ldy, const int *__restrict__ modes, int nsel,
                   const RowDesc *__restrict__ desc, int64_t n, int64_t rows_pad, int rows_per_block,
                   int64_t npts, int64_t K, const double *__restrict__ w,
-                  unsigned short *__restrict__ Xh, unsigned short *__restrict__ Xl,
-                  unsigned *__restrict__ sgn)
+                  unsigned short *__restrict__ Xh, unsigned short *__restrict__ Xl, int pb_log2)
 {
-    extern __shared__ float hsf[];  // [3][nsel][64]
-    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    extern __shared__ float hsf[];  // [3][nsel][PB]
+    const int PB = 1 << pb_log2, ng = 256 >> pb_log2;
+    const int lane = threadIdx.x & (PB - 1), grp = threadIdx.x >> pb_log2;
+    const int64_t i = (int64_t)blockIdx.x * PB + lane;
     const bool live = i < npts;
-    for (int m = grp; m < nsel; m += 4) {
+    for (int m = grp; m < nsel; m += ng) {
         const float q = live ? (float)y[(size_t)modes[m] * ldy + i] : 0.f;
-        hsf[(0 * nsel + m) * 64 + lane] = 1.41421356f * q;
-        hsf[(1 * nsel + m) * 64 + lane] = (2.f * q * q - 1.f) * 0.70710678f;
-        hsf[(2 * nsel + m) * 64 + lane] = (2.f * q * q - 3.f) * q * 0.57735027f;
-    }
-    const double wi = live ? w[i] : 0.0;
-    if (blockIdx.y == 0 && grp == 0) {
-        const unsigned long long neg = __ballot(wi < 0.0);
-        if (lane == 0) sgn[i >> 5] = (unsigned)neg;
-        if (lane == 32) sgn[i >> 5] = (unsigned)(neg >> 32);
+        hsf[(0 * nsel + m) * PB + lane] = 1.41421356f * q;
+        hsf[(1 * nsel + m) * PB + lane] = (2.f * q * q - 1.f) * 0.70710678f;
+        hsf[(2 * nsel + m) * PB + lane] = (2.f * q * q - 3.f) * q * 0.57735027f;
     }
     __syncthreads();
-    const float sw = (float)sqrt(fabs(wi));
+    const float sw = live ? (float)sqrt(fabs(w[i])) : 0.f;
     const int64_t a0 = (int64_t)blockIdx.y * rows_per_block;
     int64_t a1 = a0 + rows_per_block;
     if (a1 > rows_pad) a1 = rows_pad;
     if (i >= K) return;
     const size_t chunk = (size_t)(i >> 5) * rows_pad * 32 + (i & 31);
-    for (int64_t a = a0 + grp; a < a1; a += 4) {
+    for (int64_t a = a0 + grp; a < a1; a += ng) {
         float v = 0.f;
         if (a < n && live) {
             const RowDesc d = desc[a];
@@ -82,7 +92,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
                 const int mode = d.f[t] & 0xffff, deg = d.f[t] >> 16;
-                f[t] = deg ? hsf[((deg - 1) * nsel + mode) * 64 + lane] : 1.f;
+                f[t] = deg ? hsf[((deg - 1) * nsel + mode) * PB + lane] : 1.f;
             }
             v = sw * ((f[0] * f[1]) * f[2]);
         }
@@ -299,14 +309,18 @@ hipError_t launch_basis_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int6
 {
     const int64_t n = ctx->nbasis, rows_pad = fast_rows_pad(n);
     const int nsel = (int)ctx->modes.size();
-    const size_t smem = (size_t)3 * nsel * 64 * sizeof(float);
+    int pb_log2 = 6;
+    while (pb_log2 > 3 && (size_t)3 * nsel * (1u << pb_log2) * sizeof(float) > 64 * 1024) --pb_log2;
+    const size_t smem = (size_t)3 * nsel * (1u << pb_log2) * sizeof(float);
     if (smem > 64 * 1024) return hipErrorInvalidValue;
-    const int64_t pblocks = K / 64;
+    hipLaunchKernelGGL(k_sign_words, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, ctx->stream, w, npts, K,
+                       sgn);
+    const int64_t pblocks = K >> pb_log2;
     int rpb = 256;
     while (rpb > 16 && pblocks * ((rows_pad + rpb - 1) / rpb) < 2048) rpb /= 2;
     dim3 grid((unsigned)pblocks, (unsigned)((rows_pad + rpb - 1) / rpb));
     hipLaunchKernelGGL(k_basis_split, grid, dim3(256), smem, ctx->stream, y, ldy, ctx->d_modes, nsel,
-                       ctx->d_desc, n, rows_pad, rpb, npts, K, w, Xh, Xl, sgn);
+                       ctx->d_desc, n, rows_pad, rpb, npts, K, w, Xh, Xl, pb_log2);
     return hipGetLastError();
 }
 
