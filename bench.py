@@ -2,7 +2,7 @@
 
 Default workload (preset c2): water hexamer relaxed under q-TIP4P/F (6 H2O,
 48 modes), basis of total quanta <= 2 (1225 states), 2^20 Sobol points
-(skip 2^30) in 4096-point tiles.  One step = one full pass of the hot path
+(skip 2^30) in 131072-point tiles (the automatic tile).  One step = one full pass of the hot path
 over the 2^20 points (sample -> map -> potential -> basis -> SYRK into the
 FP64 running H) followed by the lowest 10 levels on the GPU.  Both GPU forms
 run: the plain FP64 form and the mixed form (FP32 potential and basis,
@@ -13,7 +13,7 @@ oracle path (Sobol/BSM, map, serial potential loop, get_bra, OpenBLAS
 dsyr2k) on a bounded prefix of the same points.
 
 Other presets: c1 (dimer, 91 states, 2^16 points), c3 (hexamer under
-TTM3-F, 20825 states, 2^21 points, 16384-point tiles), c4 (c3 with the MLP
+TTM3-F, 20825 states, 2^21 points, 32768-point tiles), c4 (c3 with the MLP
 control variate: 2^24 surrogate + 2^19 true points against plain FP64 at
 2^21).
 
@@ -34,10 +34,10 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
 
 PRESETS = {
-    "c1": dict(config="dimer_qtip4pf", n2=12, n3=0, points=16, tile=4096, levels=5, forms="fp64,mixed"),
-    "c2": dict(config="hexamer_qtip4pf", n2=48, n3=0, points=20, tile=4096, levels=10, forms="fp64,mixed"),
-    "c3": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=16384, levels=10, forms="fp64,mixed"),
-    "c4": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=16384, levels=10, forms="fp64,cv",
+    "c1": dict(config="dimer_qtip4pf", n2=12, n3=0, points=16, tile=0, levels=5, forms="fp64,mixed"),
+    "c2": dict(config="hexamer_qtip4pf", n2=48, n3=0, points=20, tile=0, levels=10, forms="fp64,mixed"),
+    "c3": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=0, levels=10, forms="fp64,mixed"),
+    "c4": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=0, levels=10, forms="fp64,cv",
                cv_nn=24, cv_t=19, mlp_hidden=256, mlp_steps=20000, mlp_batch=4096, mlp_pairs=18),
 }
 
@@ -108,7 +108,7 @@ def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
         t_acc += t1 - t0
         t_lev += t2 - t1
     kt = {}
-    for name in ("sobol", "map", "pes", "weight", "mlp", "basis", "syrk", "symv", "symfill"):
+    for name in ("sobol", "map", "pes", "weight", "mlp", "basis", "syrk", "syrk_mma", "symv", "symfill"):
         kt[name] = c.kernel_time(name)
     c.set_timing(False)
     npts = c.points_done()
@@ -161,8 +161,8 @@ def cpu_baseline(sysd, skip, chunk, target_s, max_points):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(DESIGN.md describes how the path would shard)" % a.gpus, file=sys.stderr)
+        print("bench.py: one GPU only; --gpus %d is not supported (the points would be split "
+              "across GPUs and the partial H summed once at the end)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     import scp3_amd
     from scp3_amd import system
@@ -188,11 +188,12 @@ def main():
     n = res[forms[0]]["n"]
     main_form = "mixed" if "mixed" in res else ("cv" if "cv" in res else forms[0])
     r = res[main_form]
-    # dominant kernel: the SYRK; algorithmic flops per launch = n(n+1) * tile points
-    ms, cnt = r["kernels"]["syrk"]
+    # dominant kernel: the SYRK's MFMA kernel (split-K partial fold timed
+    # apart); algorithmic flops of a symmetric rank-K update: n(n+1) per point
+    ms, cnt = r["kernels"]["syrk_mma"]
     avg_s = ms / 1e3 / max(cnt, 1)
-    tile = pr["tile"]
-    flops = float(n) * (n + 1) * min(tile, P)
+    pts_per_launch = r["points"] / max(cnt, 1)
+    flops = float(n) * (n + 1) * pts_per_launch
     peak = PEAK_FP64_TFLOPS if main_form == "fp64" else PEAK_BF16_TFLOPS
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
     out = {
@@ -209,11 +210,12 @@ def main():
         "dtype": "f64 H; bf16x3 MFMA SYRK, f32 potential/basis" if main_form != "fp64" else "f64",
         "data": "synthetic: seed-built relaxed cluster (scp3_amd/data), Sobol points",
         "config": {"workload": "%s: %s, %d states, 2^%d Sobol points, %d-point tiles, lowest %d levels"
-                               % (a.preset, pr["config"], n, pr["points"], tile, pr["levels"]),
+                               % (a.preset, pr["config"], n, pr["points"], round(pts_per_launch), pr["levels"]),
                    "basis_states": n, "points": P, "form": main_form},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "kernel": "syrk (%s form)" % main_form, "avg_launch_ms": avg_s * 1e3,
+                     "kernel": "%s (%s form)" % ("k_syrk_f64_lds" if main_form == "fp64" else "k_syrk_x3", main_form),
+                     "avg_launch_ms": avg_s * 1e3,
                      "flops_per_launch": flops},
         "forms": {},
     }

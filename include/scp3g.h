@@ -89,7 +89,8 @@ int scp3g_set_precision(scp3g_ctx *ctx, int mode);
 /* mixed form SYRK kernel: 1 = signed 256 x 256-tile kernel on X = sqrt|w| B
  * (default), 0 = 64 x 64-tile kernel on B and w */
 int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast);
-/* points per device tile (multiple of 256; default 4096) */
+/* points per device tile (multiple of 256), or 0 (default): the largest power
+ * of two <= 131072 whose basis table fits in 8 GiB, at least 4096 */
 int scp3g_set_tile(scp3g_ctx *ctx, int tile);
 
 /* ---- whole-path entry points -------------------------------------- */

@@ -29,6 +29,7 @@ SOURCES = [
     "context.cpp",
     "modes.cpp",
     "pes_cpu.cpp",
+    "small_eigh.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -176,12 +176,25 @@ static void free_sums(scp3g_ctx *ctx)
     ctx->h_n = 0;
 }
 
+// Points per tile when the caller sets none: per-point kernels need >= 64k
+// points to fill 256 CUs and the split-K SYRK partials amortise over the
+// tile, so take the largest power of two <= 2^17 whose FP64 table fits in
+// 8 GiB (HBM is 288 GB), but at least 4096.
+static int64_t auto_tile(int64_t n)
+{
+    const double rows = (double)rows_pad128(n) + 32;
+    int64_t T = 1 << 17;
+    while (T > 4096 && 8.0 * rows * (double)T > 8.0 * (1 << 30)) T >>= 1;
+    return T;
+}
+
 static int ensure_work(scp3g_ctx *ctx)
 {
     const int64_t n = ctx->nbasis;
-    if (ctx->work_tile == ctx->tile && ctx->work_n == n && ctx->d_B) return 0;
+    const int64_t Tw = ctx->tile > 0 ? ctx->tile : auto_tile(n);
+    if (ctx->work_tile == Tw && ctx->work_n == n && ctx->d_B) return 0;
     free_work(ctx);
-    const size_t T = (size_t)ctx->tile;
+    const size_t T = (size_t)Tw;
     const size_t rows = (size_t)rows_pad128(n);  // FP64 table rows (zero padded)
     int rc;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_y, sizeof(double) * ctx->nmodes0 * T), "alloc y")))
@@ -196,7 +209,7 @@ static int ensure_work(scp3g_ctx *ctx)
     const size_t bbytes = std::max(sizeof(double) * rows * T, 4 * rows_f * T + T / 8 + 256);
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_B, bbytes), "alloc B"))) return rc;
     if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_B, 0, bbytes, ctx->stream), "clear B"))) return rc;
-    ctx->work_tile = ctx->tile;
+    ctx->work_tile = (int)Tw;
     ctx->work_n = n;
     return 0;
 }
@@ -425,7 +438,7 @@ int scp3g_set_cv_terms(scp3g_ctx *ctx, int terms)
 int scp3g_set_tile(scp3g_ctx *ctx, int tile)
 {
     if (!ctx) return -1;
-    if (tile < 256 || tile % 256) return fail(ctx, -4, "tile must be a positive multiple of 256");
+    if (tile < 0 || tile % 256) return fail(ctx, -4, "tile must be 0 or a positive multiple of 256");
     ctx->tile = tile;
     free_work(ctx);
     return 0;
@@ -454,7 +467,7 @@ int scp3g_reset(scp3g_ctx *ctx)
 
 int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
 {
-    const int64_t T = ctx->tile;
+    const int64_t T = ctx->work_tile;
     const int fp32 = ctx->precision == SCP3G_MIXED;
     hipEvent_t ev;
     int rc;
@@ -540,7 +553,7 @@ int scp3g_accumulate_mode(scp3g_ctx *ctx, int64_t skip, int64_t npts, int wmode,
     hipSetDevice(ctx->device);
     if ((rc = ensure_sums(ctx))) return rc;
     if ((rc = ensure_work(ctx))) return rc;
-    const int64_t T = ctx->tile;
+    const int64_t T = ctx->work_tile;
     for (int64_t done = 0; done < npts; done += T) {
         const int64_t np = std::min<int64_t>(T, npts - done);
         hipEvent_t ev;
@@ -563,7 +576,7 @@ int scp3g_accumulate_normals(scp3g_ctx *ctx, const double *normals, int64_t npts
     hipSetDevice(ctx->device);
     if ((rc = ensure_sums(ctx))) return rc;
     if ((rc = ensure_work(ctx))) return rc;
-    const int64_t T = ctx->tile;
+    const int64_t T = ctx->work_tile;
     const int M = ctx->nmodes0;
     std::vector<double> yt((size_t)M * T);
     for (int64_t done = 0; done < npts; done += T) {

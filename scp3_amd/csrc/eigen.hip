@@ -24,12 +24,19 @@
 
 namespace scp3g {
 
+constexpr int kMaxB = 32;     // Davidson block size limit
+constexpr int kLdA = 32;      // A's leading dimension is a multiple of this
+constexpr int kMaxSplit = 16; // split-K limit of A X
+
+int64_t lda_of(int64_t n) { return (n + kLdA - 1) / kLdA * kLdA; }
+
 // A(r, c) = Mout(min, max) / P (+ diag on r == c) from the upper-triangle sum
 // H, through 32 x 32 LDS tiles so that both the read of the mirrored
-// triangle and the write stay coalesced.
+// triangle and the write stay coalesced.  A has leading dimension lda; its
+// padding rows n..lda-1 are left as they are (zero from allocation).
 __global__ void __launch_bounds__(256)
     k_sym_full(const double *__restrict__ H, int64_t ldh, int64_t n, double P,
-               const double *__restrict__ diag, double *__restrict__ A)
+               const double *__restrict__ diag, double *__restrict__ A, int64_t lda)
 {
     __shared__ double t[32][33];
     const int64_t bc = (int64_t)blockIdx.x * 32, br = (int64_t)blockIdx.y * 32;
@@ -42,7 +49,7 @@ __global__ void __launch_bounds__(256)
                 double v = (r <= c) ? H[(size_t)c * ldh + r] / P : 0.0;
                 if (r > c) v = H[(size_t)r * ldh + c] / P;
                 if (r == c) v += diag[r];
-                A[(size_t)c * n + r] = v;
+                A[(size_t)c * lda + r] = v;
             }
         }
     } else {
@@ -55,109 +62,197 @@ __global__ void __launch_bounds__(256)
         __syncthreads();
         for (int k = ty; k < 32; k += 8) {
             const int64_t r = br + tx, c = bc + k;
-            if (r < n && c < n) A[(size_t)c * n + r] = t[tx][k];
+            if (r < n && c < n) A[(size_t)c * lda + r] = t[tx][k];
         }
     }
 }
 
-// Y(n x b) = A X for symmetric A: Y(r, j) = sum_c A(c, r) X(c, j).
-// A block owns 64 output rows r; it streams c in chunks of 64 through LDS.
-constexpr int kMvRows = 64;
-constexpr int kMvChunk = 64;
-constexpr int kMaxB = 32;
-
+// Y(n x b) = A X for symmetric A on v_mfma_f64_16x16x4f64, computed as
+// Y^T = X^T A so that a 16-lane row of the result is 16 consecutive rows of
+// Y (128-byte stores).  A workgroup (4 waves) owns 128 rows r of Y and one
+// range of the contraction index k (split-K, partial sums to `part`); wave w
+// owns rows 32w..32w+31 (two 16-row blocks) and all NB x 16 columns.  Per
+// 32-k chunk the workgroup stages X (NB*16 columns x 32 k) in LDS and each
+// lane loads A(r, k) for its row directly: row r of A is column r (A is
+// symmetric), contiguous in k, and lda is a multiple of 32, so lane
+// (i + 16g) reads the double2 at k + 8u + 2g (u = 0..3), 64 contiguous bytes
+// per row and instruction.  MFMA step (u, v) contracts over
+// k + 8u + 2g + v, g = 0..3; both operands use that assignment.
+// The result element e of lane (i + 16g) is Y^T(4e + g, i) of the block.
+template <int NB>
 __global__ void __launch_bounds__(256)
-    k_symm_block(const double *__restrict__ A, int64_t n, const double *__restrict__ X,
-                 int64_t ldx, int b, double *__restrict__ Y, int64_t ldy)
+    k_symm_mfma(const double *__restrict__ A, int64_t lda, int64_t n, const double *__restrict__ X,
+                int64_t ldx, int b, int64_t kchunk, double *__restrict__ Y, int64_t ldy,
+                double *__restrict__ part)
 {
-    __shared__ double As[kMvChunk][kMvRows + 1];
-    __shared__ double Xs[kMvChunk][kMaxB];
-    const int64_t r0 = (int64_t)blockIdx.x * kMvRows;
-    const int ri = threadIdx.x & 63;  // output row within the block
-    const int jg = threadIdx.x >> 6;  // column group: j = jg, jg+4, ...
-    double acc[kMaxB / 4];
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    __shared__ double xs[NB * 16][34];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int64_t r0 = (int64_t)blockIdx.x * 128 + wave * 32;
+    const int split = blockIdx.y;
+    const int64_t kpad = (n + 31) / 32 * 32;
+    const int64_t kb = (int64_t)split * kchunk;
+    const int64_t ke = kb + kchunk < kpad ? kb + kchunk : kpad;
+    d4 acc[NB][2];
 #pragma unroll
-    for (int q = 0; q < kMaxB / 4; ++q) acc[q] = 0.0;
-    for (int64_t c0 = 0; c0 < n; c0 += kMvChunk) {
-        // A(c0 + cc, r0 + rr): column r0+rr is contiguous in c
-        for (int e = threadIdx.x; e < kMvChunk * kMvRows; e += 256) {
-            const int cc = e & 63, rr = e >> 6;
-            const int64_t c = c0 + cc, r = r0 + rr;
-            As[cc][rr] = (c < n && r < n) ? A[(size_t)r * n + c] : 0.0;
+    for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) acc[c][rb] = d4{0, 0, 0, 0};
+    const double *arow[2];
+    bool rok[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+        const int64_t r = r0 + 16 * rb + li;
+        rok[rb] = r < n;
+        arow[rb] = A + (size_t)(rok[rb] ? r : 0) * lda;
+    }
+    for (int64_t k = kb; k < ke; k += 32) {
+        // X chunk -> LDS: element q = tid + 256 j is column q / 32, k + q % 32
+#pragma unroll
+        for (int j = 0; j < NB * 2; ++j) {
+            const int q = tid + 256 * j, col = q >> 5, kk = q & 31;
+            xs[col][kk] = (col < b && k + kk < n) ? X[(size_t)col * ldx + k + kk] : 0.0;
         }
-        for (int e = threadIdx.x; e < kMvChunk * kMaxB; e += 256) {
-            const int cc = e % kMvChunk, j = e / kMvChunk;
-            const int64_t c = c0 + cc;
-            Xs[cc][j] = (c < n && j < b) ? X[(size_t)j * ldx + c] : 0.0;
-        }
+        double2 av[2][4];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                av[rb][u] = rok[rb] ? *reinterpret_cast<const double2 *>(arow[rb] + k + 8 * u + 2 * g)
+                                    : double2{0.0, 0.0};
         __syncthreads();
-        for (int cc = 0; cc < kMvChunk; ++cc) {
-            const double a = As[cc][ri];
 #pragma unroll
-            for (int q = 0; q < kMaxB / 4; ++q) acc[q] = fma(a, Xs[cc][jg + 4 * q], acc[q]);
+        for (int u = 0; u < 4; ++u) {
+            double2 xv[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) xv[c] = *reinterpret_cast<const double2 *>(&xs[16 * c + li][8 * u + 2 * g]);
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) {
+                    acc[c][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[c].x, av[rb][u].x, acc[c][rb], 0, 0, 0);
+                    acc[c][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[c].y, av[rb][u].y, acc[c][rb], 0, 0, 0);
+                }
         }
         __syncthreads();
     }
-    const int64_t r = r0 + ri;
-    if (r < n)
+    double *dst = part ? part + (size_t)split * kMaxB * n : Y;
+    const int64_t ld = part ? n : ldy;
 #pragma unroll
-        for (int q = 0; q < kMaxB / 4; ++q) {
-            const int j = jg + 4 * q;
-            if (j < b) Y[(size_t)j * ldy + r] = acc[q];
+    for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+            const int64_t r = r0 + 16 * rb + li;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int j = 16 * c + 4 * e + g;
+                if (j < b && r < n) dst[(size_t)j * ld + r] = acc[c][rb][e];
+            }
         }
 }
 
-// C(p x q) = P^T Q for tall P (n x p, ld ldp) and Q (n x q, ld ldq);
-// block (i, j-group of 8) reduces over n.
+// Y(r, j) = sum over splits of the partial sums, in split order.
 __global__ void __launch_bounds__(256)
-    k_gemm_tn(const double *__restrict__ Pm, int64_t ldp, const double *__restrict__ Q,
-              int64_t ldq, int64_t n, int q, double *__restrict__ C, int ldc)
+    k_symm_reduce(const double *__restrict__ part, int nsplit, int64_t n, double *__restrict__ Y,
+                  int64_t ldy)
 {
-    __shared__ double red[8][256];
-    const int i = blockIdx.x;
-    const int j0 = blockIdx.y * 8;
-    double acc[8];
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (r >= n) return;
+    double s = 0.0;
+    for (int sp = 0; sp < nsplit; ++sp) s += part[((size_t)sp * kMaxB + j) * n + r];
+    Y[(size_t)j * ldy + r] = s;
+}
+
+// C(p x q) = P^T Q for tall P (n x p, ld ldp) and Q (n x q, ld ldq).  A
+// workgroup owns a 4 x 8 block of C and reduces over n (strided by 256,
+// then across lanes and waves in a fixed order).
+__global__ void __launch_bounds__(256)
+    k_gemm_tn(const double *__restrict__ Pm, int64_t ldp, int p, const double *__restrict__ Q,
+              int64_t ldq, int q, int64_t n, double *__restrict__ C, int ldc)
+{
+    __shared__ double red[4][32];
+    const int i0 = blockIdx.x * 4, j0 = blockIdx.y * 8;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc[4][8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) acc[t] = 0.0;
-    const double *__restrict__ pc = Pm + (size_t)i * ldp;
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[a][c] = 0.0;
     for (int64_t k = threadIdx.x; k < n; k += 256) {
-        const double pv = pc[k];
+        double pv[4], qv[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
-            if (j0 + t < q) acc[t] = fma(pv, Q[(size_t)(j0 + t) * ldq + k], acc[t]);
+        for (int a = 0; a < 4; ++a) pv[a] = (i0 + a < p) ? Pm[(size_t)(i0 + a) * ldp + k] : 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) qv[c] = (j0 + c < q) ? Q[(size_t)(j0 + c) * ldq + k] : 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc[a][c] = fma(pv[a], qv[c], acc[a][c]);
     }
 #pragma unroll
-    for (int t = 0; t < 8; ++t) red[t][threadIdx.x] = acc[t];
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            double v = acc[a][c];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            acc[a][c] = v;
+        }
+    if (lane == 0)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) red[wave][a * 8 + c] = acc[a][c];
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-#pragma unroll
-            for (int t = 0; t < 8; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + s];
-        __syncthreads();
+    if (threadIdx.x < 32) {
+        const int a = threadIdx.x >> 3, c = threadIdx.x & 7;
+        const double v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+        if (i0 + a < p && j0 + c < q) C[(size_t)(j0 + c) * ldc + i0 + a] = v;
     }
-    if (threadIdx.x < 8 && j0 + (int)threadIdx.x < q)
-        C[(size_t)(j0 + threadIdx.x) * ldc + i] = red[threadIdx.x][0];
 }
 
-// Y(n x q) = beta*Y + alpha * P(n x p) S(p x q); S column-major with ld lds.
+// Y(n x q) = beta*Y + alpha * P(n x p) S(p x q), q <= kMaxB; S column-major
+// with ld lds, staged through LDS 64 rows at a time; P is read once.
 __global__ void __launch_bounds__(256)
     k_gemm_nn(const double *__restrict__ Pm, int64_t ldp, int p, const double *__restrict__ S,
               int lds, int q, double alpha, double beta, double *__restrict__ Y, int64_t ldy,
               int64_t n)
 {
+    __shared__ double ss[64][kMaxB];
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int j = blockIdx.y;
+    double acc[kMaxB];
+#pragma unroll
+    for (int j = 0; j < kMaxB; ++j) acc[j] = 0.0;
+    for (int k0 = 0; k0 < p; k0 += 64) {
+        const int kn = p - k0 < 64 ? p - k0 : 64;
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * kMaxB; e += 256) {
+            const int kk = e & 63, j = e >> 6;
+            ss[kk][j] = (kk < kn && j < q) ? S[(size_t)j * lds + k0 + kk] : 0.0;
+        }
+        __syncthreads();
+        if (r < n)
+            for (int kk = 0; kk < kn; ++kk) {
+                const double pv = Pm[(size_t)(k0 + kk) * ldp + r];
+#pragma unroll
+                for (int j = 0; j < kMaxB; ++j) acc[j] = fma(pv, ss[kk][j], acc[j]);
+            }
+    }
     if (r >= n) return;
-    double acc = 0.0;
-    for (int k = 0; k < p; ++k) acc = fma(Pm[(size_t)k * ldp + r], S[(size_t)j * lds + k], acc);
-    double *yp = Y + (size_t)j * ldy + r;
-    *yp = (beta == 0.0 ? 0.0 : beta * *yp) + alpha * acc;
+#pragma unroll
+    for (int j = 0; j < kMaxB; ++j)
+        if (j < q) {
+            double *yp = Y + (size_t)j * ldy + r;
+            *yp = (beta == 0.0 ? 0.0 : beta * *yp) + alpha * acc[j];
+        }
 }
 
-__global__ void k_diag_of(const double *__restrict__ A, int64_t n, double *__restrict__ d)
+__global__ void k_diag_of(const double *__restrict__ A, int64_t lda, int64_t n, double *__restrict__ d)
 {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (r < n) d[r] = A[(size_t)r * n + r];
+    if (r < n) d[r] = A[(size_t)r * lda + r];
 }
 
 // R = AX - X theta; T = R / (theta - diag) (Davidson correction).
@@ -219,31 +314,47 @@ namespace {
 
 struct Dav {
     scp3g_ctx *ctx;
-    int64_t n;
+    int64_t n, lda;
     const double *A;
-    double *V, *W, *X, *AX, *R, *T, *AT, *S, *G, *theta, *diag;
+    double *V, *W, *X, *AX, *R, *T, *AT, *S, *G, *theta, *diag, *part;
     int mmax, b;
 
     hipError_t symm(const double *Xp, int nb, double *Yp)
     {
         hipEvent_t ev;
         timer_begin(ctx, "symv", &ev);
-        hipLaunchKernelGGL(k_symm_block, dim3((unsigned)((n + kMvRows - 1) / kMvRows)), dim3(256), 0,
-                           ctx->stream, A, n, Xp, n, nb, Yp, n);
+        const int64_t kpad = (n + 31) / 32 * 32;
+        const unsigned rblocks = (unsigned)((n + 127) / 128);
+        // split k so the grid has ~1024 workgroups, >= 256 k per split
+        int nsplit = 1;
+        while ((int64_t)rblocks * nsplit < 1024 && kpad / (nsplit * 2) >= 256 && nsplit < kMaxSplit)
+            nsplit *= 2;
+        const int64_t kchunk = ((kpad + nsplit - 1) / nsplit + 31) / 32 * 32;
+        nsplit = (int)((kpad + kchunk - 1) / kchunk);
+        double *pp = nsplit > 1 ? part : nullptr;
+        if (nb <= 16)
+            hipLaunchKernelGGL(k_symm_mfma<1>, dim3(rblocks, (unsigned)nsplit), dim3(256), 0, ctx->stream, A,
+                               lda, n, Xp, n, nb, kchunk, Yp, n, pp);
+        else
+            hipLaunchKernelGGL(k_symm_mfma<2>, dim3(rblocks, (unsigned)nsplit), dim3(256), 0, ctx->stream, A,
+                               lda, n, Xp, n, nb, kchunk, Yp, n, pp);
+        if (pp)
+            hipLaunchKernelGGL(k_symm_reduce, dim3((unsigned)((n + 255) / 256), (unsigned)nb), dim3(256), 0,
+                               ctx->stream, pp, nsplit, n, Yp, n);
         timer_end(ctx, "symv", ev);
         return hipGetLastError();
     }
     hipError_t tn(const double *Pm, int p, const double *Q, int q, double *C, int ldc)
     {
-        hipLaunchKernelGGL(k_gemm_tn, dim3((unsigned)p, (unsigned)((q + 7) / 8)), dim3(256), 0,
-                           ctx->stream, Pm, n, Q, n, n, q, C, ldc);
+        hipLaunchKernelGGL(k_gemm_tn, dim3((unsigned)((p + 3) / 4), (unsigned)((q + 7) / 8)), dim3(256), 0,
+                           ctx->stream, Pm, n, p, Q, n, q, n, C, ldc);
         return hipGetLastError();
     }
     hipError_t nn(const double *Pm, int p, const double *Sm, int lds, int q, double alpha,
                   double beta, double *Y)
     {
-        hipLaunchKernelGGL(k_gemm_nn, dim3((unsigned)((n + 255) / 256), (unsigned)q), dim3(256), 0,
-                           ctx->stream, Pm, n, p, Sm, lds, q, alpha, beta, Y, n, n);
+        hipLaunchKernelGGL(k_gemm_nn, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, Pm, n, p,
+                           Sm, lds, q, alpha, beta, Y, n, n);
         return hipGetLastError();
     }
 };
@@ -264,9 +375,11 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
     const int b = std::min<int64_t>(k + 4, std::min<int64_t>(kMaxB, n));
     const int mmax = (int)std::min<int64_t>(n, std::max(4 * b, 64));
     const size_t nn_ = (size_t)n;
+    const int64_t lda = lda_of(n);
     double *d_A = nullptr, *d_buf = nullptr;
-    const size_t nbuf = nn_ * (2 * mmax + 5 * kMaxB) + (size_t)mmax * mmax * 2 + 2 * nn_;
-    if ((rc = check_hip(ctx, hipMalloc(&d_A, nn_ * nn_ * 8), "alloc A"))) return rc;
+    const size_t nbuf = nn_ * (2 * mmax + 5 * kMaxB) + (size_t)mmax * mmax * 2 + 2 * nn_ +
+                        (size_t)kMaxSplit * kMaxB * nn_;
+    if ((rc = check_hip(ctx, hipMalloc(&d_A, (size_t)lda * nn_ * 8), "alloc A"))) return rc;
     if ((rc = check_hip(ctx, hipMalloc(&d_buf, nbuf * 8), "alloc davidson"))) {
         hipFree(d_A);
         return rc;
@@ -274,6 +387,7 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
     Dav dv;
     dv.ctx = ctx;
     dv.n = n;
+    dv.lda = lda;
     dv.A = d_A;
     dv.mmax = mmax;
     dv.b = b;
@@ -289,6 +403,7 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
     dv.G = p; p += (size_t)mmax * mmax;
     dv.theta = p; p += nn_;
     dv.diag = p; p += nn_;
+    dv.part = p; p += (size_t)kMaxSplit * kMaxB * nn_;
 
     auto cleanup = [&]() {
         hipStreamSynchronize(ctx->stream);
@@ -304,18 +419,34 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
     } while (0)
 
     DV_CHECK(hipMemcpyAsync(dv.diag, dg.data(), nn_ * 8, hipMemcpyHostToDevice, ctx->stream), "diag");
+    // A's padding rows n..lda-1 take part in the k-chunks of A X: zero them
+    DV_CHECK(hipMemsetAsync(d_A, 0, (size_t)lda * nn_ * 8, ctx->stream), "clear A");
     {
         hipEvent_t ev;
         timer_begin(ctx, "symfill", &ev);
         dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
         hipLaunchKernelGGL(k_sym_full, grid, dim3(256), 0, ctx->stream, Hsrc, ldh, n, P, dv.diag,
-                           d_A);
+                           d_A, lda);
         timer_end(ctx, "symfill", ev);
         DV_CHECK(hipGetLastError(), "symfill");
         // the preconditioner and the start vectors use the diagonal of A itself
         hipLaunchKernelGGL(k_diag_of, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                           d_A, n, dv.diag);
+                           d_A, lda, n, dv.diag);
         DV_CHECK(hipGetLastError(), "diag");
+    }
+    if (n < 4 * b) {
+        // too small for a subspace iteration (the restarted subspace plus a
+        // block must fit in n): solve the dense problem on the host
+        std::vector<double> Ah((size_t)lda * nn_), Gd(nn_ * nn_), w, Z;
+        DV_CHECK(hipMemcpyAsync(Ah.data(), d_A, Ah.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "A");
+        DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
+        for (int64_t c = 0; c < n; ++c)
+            std::copy(Ah.begin() + (size_t)c * lda, Ah.begin() + (size_t)c * lda + n, Gd.begin() + (size_t)c * n);
+        small_eigh((int)n, Gd, k, w, Z);
+        for (int j = 0; j < k; ++j) evals[j] = w[j];
+        if (evecs) std::copy(Z.begin(), Z.begin() + nn_ * k, evecs);
+        cleanup();
+        return 0;
     }
     std::vector<double> dA(nn_);
     DV_CHECK(hipMemcpyAsync(dA.data(), dv.diag, nn_ * 8, hipMemcpyDeviceToHost, ctx->stream), "diag");
@@ -332,29 +463,40 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
                  "V0");
         DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
     }
-    int m = b;
+    // G = V^T W is kept on the host (ld mmax) and extended by the columns
+    // of each new block; after a restart it is recomputed (keep x keep).
+    int m = b, m0 = 0;
     DV_CHECK(dv.symm(dv.V, b, dv.W), "AV");
-    std::vector<double> Gh((size_t)mmax * mmax), theta, Sfull, Gm;
+    std::vector<double> Gh((size_t)mmax * mmax, 0.0), Cn, theta, Sb, Sp, Gm;
+    int sp_rows = 0;  // rows of Sp (previous Ritz coefficients), 0 = none
     std::vector<double> resn(b);
     int iter = 0;
     bool converged = false;
     const int maxiter = 400;
     for (; iter < maxiter; ++iter) {
-        // G = V^T W (m x m)
-        DV_CHECK(dv.tn(dv.V, m, dv.W, m, dv.G, m), "VtW");
-        DV_CHECK(hipMemcpyAsync(Gh.data(), dv.G, (size_t)m * m * 8, hipMemcpyDeviceToHost, ctx->stream),
-                 "G");
+        // new columns m0..m-1 of G
+        const int nc = m - m0;
+        DV_CHECK(dv.tn(dv.V, m, dv.W + (size_t)m0 * n, nc, dv.G, m), "VtW");
+        Cn.resize((size_t)m * nc);
+        DV_CHECK(hipMemcpyAsync(Cn.data(), dv.G, Cn.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "G");
         DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
-        Gm.assign(Gh.begin(), Gh.begin() + (size_t)m * m);
-        for (int c = 0; c < m; ++c)
-            for (int r = 0; r < c; ++r) {
-                const double v = 0.5 * (Gm[(size_t)c * m + r] + Gm[(size_t)r * m + c]);
-                Gm[(size_t)c * m + r] = Gm[(size_t)r * m + c] = v;
+        for (int c = 0; c < nc; ++c)
+            for (int r = 0; r < m; ++r) {
+                const double v = Cn[(size_t)c * m + r];
+                Gh[(size_t)(m0 + c) * mmax + r] = v;
+                if (r < m0) Gh[(size_t)r * mmax + m0 + c] = v;
             }
-        jacobi_eigh(m, Gm, theta, Sfull);
+        for (int c = m0; c < m; ++c)
+            for (int r = m0; r < c; ++r) {
+                const double v = 0.5 * (Gh[(size_t)c * mmax + r] + Gh[(size_t)r * mmax + c]);
+                Gh[(size_t)c * mmax + r] = Gh[(size_t)r * mmax + c] = v;
+            }
+        Gm.resize((size_t)m * m);
+        for (int c = 0; c < m; ++c)
+            std::copy(Gh.begin() + (size_t)c * mmax, Gh.begin() + (size_t)c * mmax + m, Gm.begin() + (size_t)c * m);
+        small_eigh(m, Gm, b, theta, Sb);
         // X = V S_b, AX = W S_b
-        DV_CHECK(hipMemcpyAsync(dv.S, Sfull.data(), (size_t)m * b * 8, hipMemcpyHostToDevice, ctx->stream),
-                 "S");
+        DV_CHECK(hipMemcpyAsync(dv.S, Sb.data(), (size_t)m * b * 8, hipMemcpyHostToDevice, ctx->stream), "S");
         DV_CHECK(hipMemcpyAsync(dv.theta, theta.data(), (size_t)b * 8, hipMemcpyHostToDevice, ctx->stream),
                  "theta");
         DV_CHECK(dv.nn(dv.V, m, dv.S, m, b, 1.0, 0.0, dv.X), "X");
@@ -376,12 +518,45 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
             converged = true;
             break;
         }
-        // restart if the new block does not fit
         if (m + b > mmax) {
-            DV_CHECK(hipMemcpyAsync(dv.V, dv.X, nn_ * b * 8, hipMemcpyDeviceToDevice, ctx->stream), "restart V");
-            DV_CHECK(hipMemcpyAsync(dv.W, dv.AX, nn_ * b * 8, hipMemcpyDeviceToDevice, ctx->stream), "restart W");
-            m = b;
+            // thick restart (GD+k; Stathopoulos & Saad 1998): keep the current
+            // Ritz vectors and the previous iteration's, orthonormalised in
+            // the m-dimensional coefficient space, so V <- V C, W <- W C
+            std::vector<double> C((size_t)m * kMaxB, 0.0);
+            std::copy(Sb.begin(), Sb.begin() + (size_t)m * b, C.begin());
+            int keep = b;
+            const int kprev = sp_rows > 0 ? std::min(b, kMaxB - b) : 0;
+            for (int q = 0; q < kprev; ++q) {
+                std::vector<double> x((size_t)m, 0.0);
+                for (int r = 0; r < sp_rows; ++r) x[r] = Sp[(size_t)q * sp_rows + r];
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int c = 0; c < keep; ++c) {
+                        const double *cc = &C[(size_t)c * m];
+                        double s = 0.0;
+                        for (int r = 0; r < m; ++r) s += cc[r] * x[r];
+                        for (int r = 0; r < m; ++r) x[r] -= s * cc[r];
+                    }
+                double nr = 0.0;
+                for (int r = 0; r < m; ++r) nr += x[r] * x[r];
+                nr = std::sqrt(nr);
+                if (nr < 1e-8) continue;  // no new direction
+                for (int r = 0; r < m; ++r) C[(size_t)keep * m + r] = x[r] / nr;
+                ++keep;
+            }
+            DV_CHECK(hipMemcpyAsync(dv.S, C.data(), (size_t)m * keep * 8, hipMemcpyHostToDevice, ctx->stream),
+                     "C");
+            DV_CHECK(dv.nn(dv.V, m, dv.S, m, keep, 1.0, 0.0, dv.AT), "VC");
+            DV_CHECK(hipMemcpyAsync(dv.V, dv.AT, nn_ * keep * 8, hipMemcpyDeviceToDevice, ctx->stream), "V");
+            DV_CHECK(dv.nn(dv.W, m, dv.S, m, keep, 1.0, 0.0, dv.AT), "WC");
+            DV_CHECK(hipMemcpyAsync(dv.W, dv.AT, nn_ * keep * 8, hipMemcpyDeviceToDevice, ctx->stream), "W");
+            m = keep;
+            m0 = 0;
+            sp_rows = 0;
+        } else {
+            Sp = Sb;
+            sp_rows = m;
         }
+        const bool restarted = sp_rows == 0;
         // orthogonalise T against V twice: T -= V (V^T T)
         for (int pass = 0; pass < 2; ++pass) {
             DV_CHECK(dv.tn(dv.V, m, dv.T, b, dv.G, m), "VtT");
@@ -429,6 +604,7 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
         DV_CHECK(hipMemcpyAsync(dv.V + (size_t)m * n, dv.T, nn_ * nb * 8, hipMemcpyDeviceToDevice, ctx->stream),
                  "append V");
         DV_CHECK(dv.symm(dv.V + (size_t)m * n, nb, dv.W + (size_t)m * n), "AT");
+        m0 = restarted ? 0 : m;
         m += nb;
     }
     if (!converged) {

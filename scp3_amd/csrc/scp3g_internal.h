@@ -57,7 +57,7 @@ struct scp3g_ctx {
     std::vector<scp3g::RowDesc> desc;
     int pes = SCP3G_PES_QTIP4PF;
     int precision = SCP3G_FP64;
-    int tile = 4096;
+    int tile = 0;  // points per tile; 0 = automatic (auto_tile)
     // LDS-tiled SYRK kernels (syrk_fast.hip: signed 256 x 256 split-BF16;
     // syrk64_fast.hip: 128 x 128 FP64) instead of the 64 x 64 direct-load
     // kernels (syrk.hip, syrk_mixed.hip)
@@ -154,6 +154,9 @@ void mlp_release(scp3g_ctx *ctx);
 // host helpers
 std::vector<double> ho_diagonal(const scp3g_ctx *ctx);  // HO diagonal + unselected ZPE
 void sobol_directions(int dims, uint64_t *v);  // dims x 62, scaled
+// lowest nvec eigenpairs of a small dense symmetric matrix (column-major,
+// both triangles, overwritten); w ascending, Z m x nvec (small_eigh.cpp)
+int small_eigh(int m, std::vector<double> &A, int nvec, std::vector<double> &w, std::vector<double> &Z);
 int jacobi_eigh(int n, std::vector<double> &A, std::vector<double> &w,
                 std::vector<double> &V);
 

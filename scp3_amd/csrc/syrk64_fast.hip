@@ -190,8 +190,11 @@ hipError_t launch_syrk_f64_lds(scp3g_ctx *ctx, int64_t n, const double *B, int64
         }
         part = ctx->d_part;
     }
+    hipEvent_t ev;
+    timer_begin(ctx, "syrk_mma", &ev);
     hipLaunchKernelGGL(k_syrk_f64_lds, dim3((unsigned)ntiles, (unsigned)nsplit), dim3(256), 0, ctx->stream, B,
                        ldb, w, tc.d, ntiles, kchunk, kend, n, H, part);
+    timer_end(ctx, "syrk_mma", ev);
     e = hipGetLastError();
     if (e != hipSuccess || !part) return e;
     hipLaunchKernelGGL(k_syrk_f64_lds_reduce, dim3((unsigned)ntiles, 8), dim3(256), 0, ctx->stream, part, tc.d,

@@ -350,6 +350,8 @@ hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
         part = reinterpret_cast<float *>(ctx->d_part);
     }
     const int64_t rows_pad = fast_rows_pad(n);
+    hipEvent_t ev;
+    timer_begin(ctx, "syrk_mma", &ev);
     if (terms == 1)
         hipLaunchKernelGGL(k_syrk_x3<1>, dim3((unsigned)ntiles, (unsigned)nsplit), dim3(512), 0,
                            ctx->stream, reinterpret_cast<const uint4 *>(Xh),
@@ -360,6 +362,7 @@ hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
                            ctx->stream, reinterpret_cast<const uint4 *>(Xh),
                            reinterpret_cast<const uint4 *>(Xl), sgn, rows_pad, tiles, ntiles, kbs, nkb, n,
                            H, Hc, part);
+    timer_end(ctx, "syrk_mma", ev);
     e = hipGetLastError();
     if (e != hipSuccess || !part) return e;
     hipLaunchKernelGGL(k_syrk_x3_fold, dim3((unsigned)ntiles, 16), dim3(256), 0, ctx->stream, part, tiles,

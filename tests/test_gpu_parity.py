@@ -98,12 +98,13 @@ def test_map_and_qtip4pf(ctx, hexamer, fp32):
         # Rounding the coordinates to FP32 alone moves E by up to ~5e-4
         # kcal/mol on this cluster (oracle on rounded vs exact coordinates),
         # without bias.  Bounds: end to end 1e-3 kcal/mol (< 1.4e-3 = 0.5
-        # cm^-1) with |mean| < 2e-5; the kernel's own FP32 arithmetic, on the
-        # same FP32 coordinates, 2e-4.
+        # cm^-1) with |mean| < 2e-5; the kernel's own FP32 arithmetic (135
+        # Coulomb terms of O(100) kcal/mol, summed in double), on the same
+        # FP32 coordinates, 5e-4.
         d = Eg - E_ref
         assert np.abs(d).max() < 1e-3 and abs(d.mean()) < 2e-5, (np.abs(d).max(), d.mean())
         E_own = np.array([oracle.qtip4pf(r[i]) for i in range(P)])
-        assert np.abs(Eg - E_own).max() < 2e-4, np.abs(Eg - E_own).max()
+        assert np.abs(Eg - E_own).max() < 5e-4, np.abs(Eg - E_own).max()
     else:
         np.testing.assert_allclose(r, r_ref, rtol=1e-13, atol=1e-13)
         np.testing.assert_allclose(Eg, E_ref, rtol=1e-12, atol=1e-11)
