@@ -192,7 +192,7 @@ def main():
     # apart); algorithmic flops of a symmetric rank-K update: n(n+1) per point
     ms, cnt = r["kernels"]["syrk_mma"]
     avg_s = ms / 1e3 / max(cnt, 1)
-    pts_per_launch = r["points"] / max(cnt, 1)
+    pts_per_launch = r["points"] * a.steps / max(cnt, 1)  # points_done counts one step
     flops = float(n) * (n + 1) * pts_per_launch
     peak = PEAK_FP64_TFLOPS if main_form == "fp64" else PEAK_BF16_TFLOPS
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
