@@ -42,37 +42,51 @@ __global__ void __launch_bounds__(256, 2)
         for (int b = 0; b < 4; ++b) acc[a][b] = d4{0, 0, 0, 0};
 
     // loader: 128 rows x 16 points per operand = 1024 double2; thread tid
-    // moves double2 q = tid + 256 j (row q / 8, points 2 (q % 8) .. +1)
-    double2 sa[4], sb[4];
-    auto gload = [&](int64_t p0) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int q = tid + 256 * j, r = q >> 3, c = (q & 7) * 2;
-            sa[j] = *reinterpret_cast<const double2 *>(B + (size_t)(rA + r) * ldb + p0 + c);
-            double2 v = *reinterpret_cast<const double2 *>(B + (size_t)(rB + r) * ldb + p0 + c);
-            const double2 ww = *reinterpret_cast<const double2 *>(w + p0 + c);
-            v.x *= ww.x;
-            v.y *= ww.y;
-            sb[j] = v;
-        }
-    };
-    auto lstore = [&](int s) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int q = tid + 256 * j, r = q >> 3, c = (q & 7) * 2;
-            *reinterpret_cast<double2 *>(&lds[s][0][r][c]) = sa[j];
-            *reinterpret_cast<double2 *>(&lds[s][1][r][c]) = sb[j];
-        }
-    };
+    // moves double2 q = tid + 256 j (row q / 8, points 2 (q % 8) .. +1),
+    // j = 0..3, into named registers (an array here would be promoted to LDS
+    // scratch).  Step p+16 is fetched before step p's MFMAs and stored after
+    // them; the load is unconditional (the last step re-reads its own
+    // points into the idle buffer) and sched_barriers keep the order.
+    const int lr = tid >> 3, lc = (tid & 7) * 2;  // row and point of j = 0
+    double2 sa0, sa1, sa2, sa3, sb0, sb1, sb2, sb3;
+    double2 ww;  // the step's weights for this thread's two points
+#define F64_GLOAD1(j, sa, sb, p0)                                                                     \
+    do {                                                                                              \
+        sa = *reinterpret_cast<const double2 *>(B + (size_t)(rA + lr + 32 * (j)) * ldb + (p0) + lc); \
+        sb = *reinterpret_cast<const double2 *>(B + (size_t)(rB + lr + 32 * (j)) * ldb + (p0) + lc); \
+    } while (0)
+#define F64_GLOAD(p0)                                                  \
+    do {                                                               \
+        F64_GLOAD1(0, sa0, sb0, p0);                                   \
+        F64_GLOAD1(1, sa1, sb1, p0);                                   \
+        F64_GLOAD1(2, sa2, sb2, p0);                                   \
+        F64_GLOAD1(3, sa3, sb3, p0);                                   \
+        ww = *reinterpret_cast<const double2 *>(w + (p0) + lc);        \
+    } while (0)
+    // the weight is applied to the right operand on its way into LDS
+#define F64_STB(sg, rr, sb) \
+    *reinterpret_cast<double2 *>(&lds[sg][1][rr][lc]) = double2{(sb).x * ww.x, (sb).y * ww.y}
+#define F64_LSTORE(sg)                                                                  \
+    do {                                                                                \
+        *reinterpret_cast<double2 *>(&lds[sg][0][lr][lc]) = sa0;                        \
+        *reinterpret_cast<double2 *>(&lds[sg][0][lr + 32][lc]) = sa1;                   \
+        *reinterpret_cast<double2 *>(&lds[sg][0][lr + 64][lc]) = sa2;                   \
+        *reinterpret_cast<double2 *>(&lds[sg][0][lr + 96][lc]) = sa3;                   \
+        F64_STB(sg, lr, sb0);                                                           \
+        F64_STB(sg, lr + 32, sb1);                                                      \
+        F64_STB(sg, lr + 64, sb2);                                                      \
+        F64_STB(sg, lr + 96, sb3);                                                      \
+    } while (0)
     if (kb < ke) {
-        gload(kb);
-        lstore(0);
+        F64_GLOAD(kb);
+        F64_LSTORE(0);
     }
     __syncthreads();
     int s = 0;
     for (int64_t p = kb; p < ke; p += 16) {
-        const bool more = p + 16 < ke;
-        if (more) gload(p + 16);
+        const int64_t pn = p + 16 < ke ? p + 16 : p;
+        F64_GLOAD(pn);
+        __builtin_amdgcn_sched_barrier(0);
         d4 av[4], bv[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -91,10 +105,15 @@ __global__ void __launch_bounds__(256, 2)
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a][k], bv[b][k], acc[a][b], 0, 0, 0);
-        if (more) lstore(s ^ 1);
+        __builtin_amdgcn_sched_barrier(0);
+        F64_LSTORE(s ^ 1);
         __syncthreads();
         s ^= 1;
     }
+#undef F64_GLOAD1
+#undef F64_GLOAD
+#undef F64_STB
+#undef F64_LSTORE
 
     // v_mfma_f64_16x16x4f64 result: D[i][j] sits in lane j + 16*(i%4),
     // element i/4 (rows interleave across the four lane groups)

@@ -139,41 +139,53 @@ __global__ void __launch_bounds__(512, 1)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 
-    // each thread moves 2 chunks of each plane per step (chunk q = tid + 512 j,
-    // q = row * 4 + chunk); the A planes are staged while the first half of
-    // the column blocks is computed and the B planes during the second half
-    uint4 st[4];
-    auto gload = [&](int kb, size_t row0) {
-        const size_t base = (size_t)kb * planeStride + row0 * 4;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int q = tid + 512 * j;
-            st[j] = Xh[base + q];
-            st[2 + j] = Xl[base + q];
-        }
-    };
-    auto lstore = [&](int sg, int plane0) {
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int q = tid + 512 * j;
-            const int r = q >> 2, c = q & 3;
-            const int p = r * 4 + (c ^ ((r >> 2) & 3));
-            lds[sg][plane0][p] = st[j];
-            lds[sg][plane0 + 1][p] = st[2 + j];
-        }
-    };
+    // Staging: each thread moves chunks q0 = tid and q1 = tid + 512 (q = row
+    // * 4 + chunk) of each plane per step into named registers (an array
+    // here would be promoted to LDS scratch).  The A planes of step kb+1 are
+    // fetched before the first half of the column blocks is computed and
+    // stored after it, the B planes likewise around the second half; the
+    // loads are unconditional (the last step re-reads its own chunk into the
+    // idle buffer) and sched_barriers keep the stores, with their wait for
+    // memory, behind the MFMAs.  TERMS = 1 never touches the lo planes.
+    const int q0 = tid, q1 = tid + 512;
+    const int p0 = (q0 >> 2) * 4 + ((q0 & 3) ^ ((q0 >> 4) & 3));
+    const int p1 = (q1 >> 2) * 4 + ((q1 & 3) ^ ((q1 >> 4) & 3));
+    uint4 sh0, sh1, sl0, sl1;
+#define X3_GLOAD(kbv, row0)                                                   \
+    do {                                                                      \
+        const size_t base_ = (size_t)(kbv) * planeStride + (row0) * 4;        \
+        sh0 = Xh[base_ + q0];                                                 \
+        sh1 = Xh[base_ + q1];                                                 \
+        if (TERMS == 3) {                                                     \
+            sl0 = Xl[base_ + q0];                                             \
+            sl1 = Xl[base_ + q1];                                             \
+        }                                                                     \
+    } while (0)
+#define X3_LSTORE(sg, plane0)                                                 \
+    do {                                                                      \
+        lds[sg][plane0][p0] = sh0;                                            \
+        lds[sg][plane0][p1] = sh1;                                            \
+        if (TERMS == 3) {                                                     \
+            lds[sg][(plane0) + 1][p0] = sl0;                                  \
+            lds[sg][(plane0) + 1][p1] = sl1;                                  \
+        }                                                                     \
+    } while (0)
     if (kb0 < kb1) {
-        gload(kb0, rowA);
-        lstore(0, 0);
-        gload(kb0, rowB);
-        lstore(0, 2);
+        X3_GLOAD(kb0, rowA);
+        X3_LSTORE(0, 0);
+        X3_GLOAD(kb0, rowB);
+        X3_LSTORE(0, 2);
     }
     __syncthreads();
     int s = 0;
+    unsigned sw_next = kb0 < kb1 ? sgn[kb0] : 0u;
     for (int kb = kb0; kb < kb1; ++kb) {
-        const bool more = kb + 1 < kb1;
-        // sign flips for this lane's 8 points (8 lg .. 8 lg + 7)
-        const unsigned bits = (sgn[kb] >> (8 * lg)) & 0xffu;
+        const int kn = kb + 1 < kb1 ? kb + 1 : kb;
+        // sign flips for this lane's 8 points (8 lg .. 8 lg + 7); the next
+        // step's word is fetched a step ahead
+        const unsigned sw = sw_next;
+        sw_next = sgn[kn];
+        const unsigned bits = (sw >> (8 * lg)) & 0xffu;
         uint4 fl;
         fl.x = ((bits & 1u) << 15) | (((bits >> 1) & 1u) << 31);
         fl.y = (((bits >> 2) & 1u) << 15) | (((bits >> 3) & 1u) << 31);
@@ -184,36 +196,54 @@ __global__ void __launch_bounds__(512, 1)
         for (int a = 0; a < 8; ++a) {
             const int r = wr * 128 + 16 * a + li;
             const int p = r * 4 + (lg ^ ((r >> 2) & 3));
-            uint4 vh = lds[s][0][p], vl = lds[s][1][p];
+            uint4 vh = lds[s][0][p];
             vh.x ^= fl.x; vh.y ^= fl.y; vh.z ^= fl.z; vh.w ^= fl.w;
-            vl.x ^= fl.x; vl.y ^= fl.y; vl.z ^= fl.z; vl.w ^= fl.w;
             ah[a] = __builtin_bit_cast(bf16x8, vh);
-            al[a] = __builtin_bit_cast(bf16x8, vl);
+            if (TERMS == 3) {
+                uint4 vl = lds[s][1][p];
+                vl.x ^= fl.x; vl.y ^= fl.y; vl.z ^= fl.z; vl.w ^= fl.w;
+                al[a] = __builtin_bit_cast(bf16x8, vl);
+            }
         }
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
-            if (more) gload(kb + 1, half ? rowB : rowA);
+            if (half == 0)
+                X3_GLOAD(kn, rowA);
+            else
+                X3_GLOAD(kn, rowB);
+            __builtin_amdgcn_sched_barrier(0);
+            bf16x8 bh[2], bl[2];
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb) {
-                const int b = 2 * half + bb;
-                const int r = wc * 64 + 16 * b + li;
+                const int r = wc * 64 + 16 * (2 * half + bb) + li;
                 const int p = r * 4 + (lg ^ ((r >> 2) & 3));
-                const bf16x8 bh = __builtin_bit_cast(bf16x8, lds[s][2][p]);
-                const bf16x8 bl = __builtin_bit_cast(bf16x8, lds[s][3][p]);
-#pragma unroll
-                for (int a = 0; a < 8; ++a) {
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
-                    if (TERMS == 3) {
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
-                    }
-                }
+                bh[bb] = __builtin_bit_cast(bf16x8, lds[s][2][p]);
+                if (TERMS == 3) bl[bb] = __builtin_bit_cast(bf16x8, lds[s][3][p]);
             }
-            if (more) lstore(s ^ 1, half ? 2 : 0);
+            // term-major order: 16 independent accumulators between two
+            // updates of the same one
+#pragma unroll
+            for (int term = 0; term < TERMS; ++term)
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+#pragma unroll
+                    for (int bb = 0; bb < 2; ++bb) {
+                        const bf16x8 x = term == 2 ? al[a] : ah[a];
+                        const bf16x8 y = term == 1 ? bl[bb] : bh[bb];
+                        acc[a][2 * half + bb] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, acc[a][2 * half + bb], 0, 0, 0);
+                    }
+            __builtin_amdgcn_sched_barrier(0);
+            if (half == 0)
+                X3_LSTORE(s ^ 1, 0);
+            else
+                X3_LSTORE(s ^ 1, 2);
         }
         __syncthreads();
         s ^= 1;
     }
+#undef X3_GLOAD
+#undef X3_LSTORE
 
     // D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4
     const int oj = lane & 15, oi = 4 * (lane >> 4);
