@@ -54,7 +54,22 @@ def parse():
     ap.add_argument("--cpu-chunk", type=int, default=8192, help="points per CPU baseline chunk")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cv-terms", type=int, default=1)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    """One plain device-to-device copy of nbytes (torch's copy kernel): the
+    known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "check": float(dst[-1].item())}))
 
 
 def setup(c, sysd, pr, mode):
@@ -160,6 +175,9 @@ def cpu_baseline(sysd, skip, chunk, target_s, max_points):
 
 def main():
     a = parse()
+    if a.copy_bytes:
+        copy_calibration(a.copy_bytes)
+        return
     if a.gpus != 1:
         print("bench.py: one GPU only; --gpus %d is not supported (the points would be split "
               "across GPUs and the partial H summed once at the end)" % a.gpus, file=sys.stderr)

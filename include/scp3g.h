@@ -182,6 +182,10 @@ int scp3g_accumulate_cv(scp3g_ctx *ctx, int64_t skip_nn, int64_t P_nn, int64_t s
 /* mixed form: split-BF16 terms used for the surrogate term of the control
  * variate, 3 (default, hi.hi + hi.lo + lo.hi) or 1 (hi.hi) */
 int scp3g_set_cv_terms(scp3g_ctx *ctx, int terms);
+/* the surrogate term alone: H += sum B B^T w_nn over Sobol seeds
+ * skip..skip+npts-1, normaliser += npts (QMC with V_nn in place of V, the
+ * first term of scp3g_accumulate_cv); no reference counterpart */
+int scp3g_accumulate_surrogate(scp3g_ctx *ctx, int64_t skip, int64_t npts);
 
 /* ---- per-stage entry points (device pointers; used by parity tests) ---- */
 

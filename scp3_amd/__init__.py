@@ -44,6 +44,7 @@ _SIGS = {
     "scp3g_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_mixed_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_cv_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "scp3g_accumulate_surrogate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "scp3g_reset": (ctypes.c_int, [ctypes.c_void_p]),
     "scp3g_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "scp3g_accumulate_normals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
@@ -275,6 +276,12 @@ class Context:
 
     def accumulate_cv(self, skip_nn, P_nn, skip_t, P_t):
         self.check(self.L.scp3g_accumulate_cv(self._h, int(skip_nn), int(P_nn), int(skip_t), int(P_t)))
+
+    def accumulate_surrogate(self, skip, npts):
+        self.check(self.L.scp3g_accumulate_surrogate(self._h, int(skip), int(npts)))
+
+    def set_cv_terms(self, terms):
+        self.check(self.L.scp3g_set_cv_terms(self._h, int(terms)))
 
     # timing -------------------------------------------------------------
     def set_timing(self, on):

@@ -792,6 +792,14 @@ int scp3g_mlp_eval(scp3g_ctx *ctx, const double *y_dev, int64_t ldy, int64_t npt
 // skip_nn .. skip_nn+P_nn-1, each scaled 1/P_nn, plus the correction
 // bV - w_nn over seeds skip_t .. skip_t+P_t-1, scaled 1/P_t.  The running
 // sum then normalises to 1 (scp3g_h_download divides by it).
+int scp3g_accumulate_surrogate(scp3g_ctx *ctx, int64_t skip, int64_t npts)
+{
+    int rc = mlp_ready(ctx);
+    if (rc) return rc;
+    if (npts < 0 || skip < 0) return fail(ctx, -4, "negative skip or point count");
+    return scp3g_accumulate_mode(ctx, skip, npts, 1, 1.0);
+}
+
 int scp3g_accumulate_cv(scp3g_ctx *ctx, int64_t skip_nn, int64_t P_nn, int64_t skip_t, int64_t P_t)
 {
     int rc = mlp_ready(ctx);

@@ -144,3 +144,52 @@ def test_control_variate_beats_plain_qmc(gpu_ctx_factory, hexamer):
           % (loss[-100:].mean(), e_pl, e_cv, l_pl, l_cv))
     assert e_cv < e_pl
     assert l_cv < l_pl
+
+
+def _bf16_rne(x32):
+    u = x32.astype(np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("terms", [1, 3])
+def test_surrogate_term_split_bf16_products(gpu_ctx_factory, hexamer, terms):
+    """The surrogate term on the mixed form's signed split-BF16 SYRK with a
+    constant surrogate w_nn = c < 0 (all sign bits set): against the
+    hi.hi-only product computed on the host (terms = 1) or the FP64 product
+    (terms = 3), on the same Sobol points."""
+    import scp3_amd
+    from oracle import oracle
+    print("surrogate term case: terms=%d" % terms, flush=True)
+    ctx = gpu_ctx_factory(0)
+    _small_basis(ctx, hexamer, scp3_amd.MIXED)
+    ctx.set_tile(0)
+    ctx.set_cv_terms(terms)
+    ctx.mlp_init(64, seed=1)
+    P0, _ = ctx.mlp_params()
+    c = -0.0123
+    ctx.mlp_set_params(np.zeros_like(P0), np.array([c, 0.0, 1.0]))   # f = 0, w_nn = c
+    skip, P = 1 << 30, 3 * 4096 + 777
+    ctx.reset()
+    ctx.accumulate_surrogate(skip, P)
+    H = ctx.h_download(raw=True)
+    y = oracle.sobol_normals(48, skip, P)
+    b = oracle.get_bra(y[:, hexamer["modes"]], 48, 12, 0)          # P x n, FP64
+    H64 = c * (b.T @ b)
+    x = np.float32(np.sqrt(abs(c))) * b.astype(np.float32)
+    hi = _bf16_rne(x).astype(np.float64)
+    H1 = np.sign(c) * (hi.T @ hi)
+    iu = np.triu_indices(b.shape[1])
+    err1 = np.abs(H - H1)[iu].max()
+    err64 = np.abs(H - H64)[iu].max()
+    model = np.abs(H1 - H64)[iu].max()
+    print("surrogate term terms=%d: |H-H(hi.hi)| %.3e  |H-H64| %.3e  |H(hi.hi)-H64| %.3e  max|H| %.3e"
+          % (terms, err1, err64, model, np.abs(H64).max()))
+    if terms == 1:
+        # the device computes exactly the hi.hi product (up to FP32 sums and
+        # rare one-ulp differences of the FP32 basis values)
+        assert err1 < 0.02 * model
+    else:
+        assert err64 < 1e-5 * np.abs(H64).max()
+        assert err64 < 0.02 * model
