@@ -116,14 +116,16 @@ __device__ __forceinline__ void kahan_add2(double &h, double &c, double v)
 template <int TERMS>
 __global__ void __launch_bounds__(512, 1)
     k_syrk_x3(const uint4 *__restrict__ Xh, const uint4 *__restrict__ Xl,
-              const unsigned *__restrict__ sgn, int64_t rows_pad, const int2 *__restrict__ tiles,
+              const unsigned *__restrict__ sgn, int64_t rows_pad, const int4 *__restrict__ work,
               int ntiles, int kb_per_split, int nkb, int64_t n, double *__restrict__ H,
               double *__restrict__ Hc, float *__restrict__ part)
 {
     // [stage][plane Ah, Al, Bh, Bl][row * 4 + swizzled chunk]
     __shared__ uint4 lds[2][4][kFT * 4];
-    const int2 t = tiles[blockIdx.x];
-    const int split = blockIdx.y;
+    // work item: tile (row block, column block), split, canonical tile index
+    const int4 wk = work[blockIdx.x];
+    const int2 t = make_int2(wk.x, wk.y);
+    const int split = wk.z, tile = wk.w;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 2, wc = wave & 3;
     const int li = lane & 15, lg = lane >> 4;
@@ -260,7 +262,7 @@ __global__ void __launch_bounds__(512, 1)
                 const int64_t row = (int64_t)rowA + tr;
                 const float v = acc[a][b][e];
                 if (part) {
-                    part[((size_t)split * ntiles + blockIdx.x) * (kFT * kFT) + (size_t)tr * kFT + tc] = v;
+                    part[((size_t)split * ntiles + tile) * (kFT * kFT) + (size_t)tr * kFT + tc] = v;
                 } else if (row <= col && col < n) {
                     const size_t idx = (size_t)col * n + row;
                     double h = H[idx], c = Hc[idx];
@@ -272,27 +274,48 @@ __global__ void __launch_bounds__(512, 1)
         }
 }
 
+// One thread per tile element: the split-K partials are summed in FP64 in a
+// fixed order (four interleaved partial sums, then (s0 + s1) + (s2 + s3)) so
+// that the loads of several splits are in flight at once, and the sum is
+// Kahan-added into H.
 __global__ void __launch_bounds__(256)
     k_syrk_x3_fold(const float *__restrict__ part, const int2 *__restrict__ tiles, int ntiles,
                    int nsplit, int64_t n, double *__restrict__ H, double *__restrict__ Hc)
 {
     const int tile = blockIdx.x;
     const int2 t = tiles[tile];
-    for (int e = blockIdx.y * 256 + threadIdx.x; e < kFT * kFT; e += 256 * gridDim.y) {
-        const int tr = e / kFT, tc = e % kFT;
-        const int64_t row = (int64_t)t.x * kFT + tr, col = (int64_t)t.y * kFT + tc;
-        if (row > col || col >= n) continue;
-        double s = 0.0;
-        for (int sp = 0; sp < nsplit; ++sp) s += (double)part[((size_t)sp * ntiles + tile) * (kFT * kFT) + e];
-        const size_t idx = (size_t)col * n + row;
-        double h = H[idx], c = Hc[idx];
-        kahan_add2(h, c, s);
-        H[idx] = h;
-        Hc[idx] = c;
+    const int e = blockIdx.y * 256 + threadIdx.x;
+    const int tr = e / kFT, tc = e % kFT;
+    const int64_t row = (int64_t)t.x * kFT + tr, col = (int64_t)t.y * kFT + tc;
+    if (row > col || col >= n) return;
+    const size_t stride = (size_t)ntiles * (kFT * kFT);
+    const float *p = part + (size_t)tile * (kFT * kFT) + e;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int sp = 0;
+    for (; sp + 4 <= nsplit; sp += 4) {
+        s0 += (double)p[(size_t)sp * stride];
+        s1 += (double)p[(size_t)(sp + 1) * stride];
+        s2 += (double)p[(size_t)(sp + 2) * stride];
+        s3 += (double)p[(size_t)(sp + 3) * stride];
     }
+    for (; sp < nsplit; ++sp) s0 += (double)p[(size_t)sp * stride];
+    const double s = (s0 + s1) + (s2 + s3);
+    const size_t idx = (size_t)col * n + row;
+    double h = H[idx], c = Hc[idx];
+    kahan_add2(h, c, s);
+    H[idx] = h;
+    Hc[idx] = c;
 }
 
 // --------------------------------------------------------------------------
+
+struct WorkOrder {
+    int64_t n = -1;
+    int nsplit = 0;
+    int4 *d = nullptr;
+    int count = 0;
+};
+static std::map<scp3g_ctx *, WorkOrder> g_work;
 
 struct FastTiles {
     int64_t n = -1;
@@ -325,6 +348,11 @@ static hipError_t fast_tiles(scp3g_ctx *ctx, int64_t n, const int2 **tiles, int 
 
 void release_fast_tiles(scp3g_ctx *ctx)
 {
+    auto iw = g_work.find(ctx);
+    if (iw != g_work.end()) {
+        if (iw->second.d) hipFree(iw->second.d);
+        g_work.erase(iw);
+    }
     auto it = g_ftiles.find(ctx);
     if (it != g_ftiles.end()) {
         if (it->second.d) hipFree(it->second.d);
@@ -354,6 +382,63 @@ hipError_t launch_basis_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int6
     return hipGetLastError();
 }
 
+// Dispatch order of the (tile, split) work items.  Consecutive workgroups
+// go to the 8 XCDs in turn and each XCD has its own L2, so the items are
+// first listed in an order where neighbours share operand rows (split-major
+// when the K range is split: the tiles of one split read the same few row
+// blocks of that range; otherwise 4 x 8 super-tiles of the triangle: 32
+// tiles reading 12 row blocks), the list is cut into 8 contiguous chunks,
+// and chunk x is dealt to the workgroup ids congruent to x mod 8.
+
+static hipError_t work_order(scp3g_ctx *ctx, int64_t n, int nsplit, const int4 **work, int *count)
+{
+    WorkOrder &wo = g_work[ctx];
+    if (wo.n != n || wo.nsplit != nsplit) {
+        if (wo.d) hipFree(wo.d);
+        wo.d = nullptr;
+        wo.n = -1;
+        const int T = (int)((n + kFT - 1) / kFT);
+        const int ntiles = T * (T + 1) / 2;
+        std::vector<int4> grouped;
+        grouped.reserve((size_t)ntiles * nsplit);
+        if (nsplit > 1) {
+            for (int sp = 0; sp < nsplit; ++sp)
+                for (int j = 0; j < T; ++j)
+                    for (int i = 0; i <= j; ++i) grouped.push_back(make_int4(i, j, sp, j * (j + 1) / 2 + i));
+        } else {
+            for (int J0 = 0; J0 < T; J0 += 8)
+                for (int I0 = 0; I0 < std::min(J0 + 8, T); I0 += 4)
+                    for (int j = J0; j < std::min(J0 + 8, T); ++j)
+                        for (int i = I0; i < std::min(I0 + 4, j + 1); ++i)
+                            grouped.push_back(make_int4(i, j, 0, j * (j + 1) / 2 + i));
+        }
+        const int total = (int)grouped.size();
+        if (total != ntiles * nsplit) return hipErrorInvalidValue;
+        const int q = total / 8, r = total % 8;
+        std::vector<int4> v((size_t)total);
+        std::vector<char> seen((size_t)total, 0);
+        for (int x = 0; x < 8; ++x) {
+            const int start = x * q + std::min(x, r), len = q + (x < r ? 1 : 0);
+            for (int k = 0; k < len; ++k) {
+                const int L = 8 * k + x;
+                if (L >= total || seen[L]) return hipErrorInvalidValue;
+                seen[L] = 1;
+                v[L] = grouped[start + k];
+            }
+        }
+        hipError_t e = hipMalloc(&wo.d, v.size() * sizeof(int4));
+        if (e != hipSuccess) return e;
+        e = hipMemcpy(wo.d, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return e;
+        wo.n = n;
+        wo.nsplit = nsplit;
+        wo.count = total;
+    }
+    *work = wo.d;
+    *count = wo.count;
+    return hipSuccess;
+}
+
 hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
                           const unsigned short *Xl, const unsigned *sgn, int64_t K, double *H,
                           double *Hc, int terms)
@@ -380,23 +465,24 @@ hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
         part = reinterpret_cast<float *>(ctx->d_part);
     }
     const int64_t rows_pad = fast_rows_pad(n);
+    const int4 *work;
+    int nwork;
+    if ((e = work_order(ctx, n, nsplit, &work, &nwork)) != hipSuccess) return e;
     hipEvent_t ev;
     timer_begin(ctx, "syrk_mma", &ev);
     if (terms == 1)
-        hipLaunchKernelGGL(k_syrk_x3<1>, dim3((unsigned)ntiles, (unsigned)nsplit), dim3(512), 0,
-                           ctx->stream, reinterpret_cast<const uint4 *>(Xh),
-                           reinterpret_cast<const uint4 *>(Xl), sgn, rows_pad, tiles, ntiles, kbs, nkb, n,
-                           H, Hc, part);
+        hipLaunchKernelGGL(k_syrk_x3<1>, dim3((unsigned)nwork), dim3(512), 0, ctx->stream,
+                           reinterpret_cast<const uint4 *>(Xh), reinterpret_cast<const uint4 *>(Xl), sgn,
+                           rows_pad, work, ntiles, kbs, nkb, n, H, Hc, part);
     else
-        hipLaunchKernelGGL(k_syrk_x3<3>, dim3((unsigned)ntiles, (unsigned)nsplit), dim3(512), 0,
-                           ctx->stream, reinterpret_cast<const uint4 *>(Xh),
-                           reinterpret_cast<const uint4 *>(Xl), sgn, rows_pad, tiles, ntiles, kbs, nkb, n,
-                           H, Hc, part);
+        hipLaunchKernelGGL(k_syrk_x3<3>, dim3((unsigned)nwork), dim3(512), 0, ctx->stream,
+                           reinterpret_cast<const uint4 *>(Xh), reinterpret_cast<const uint4 *>(Xl), sgn,
+                           rows_pad, work, ntiles, kbs, nkb, n, H, Hc, part);
     timer_end(ctx, "syrk_mma", ev);
     e = hipGetLastError();
     if (e != hipSuccess || !part) return e;
-    hipLaunchKernelGGL(k_syrk_x3_fold, dim3((unsigned)ntiles, 16), dim3(256), 0, ctx->stream, part, tiles,
-                       ntiles, nsplit, n, H, Hc);
+    hipLaunchKernelGGL(k_syrk_x3_fold, dim3((unsigned)ntiles, kFT * kFT / 256), dim3(256), 0, ctx->stream, part,
+                       tiles, ntiles, nsplit, n, H, Hc);
     return hipGetLastError();
 }
 
