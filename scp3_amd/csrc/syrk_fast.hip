@@ -111,6 +111,195 @@ __device__ __forceinline__ void kahan_add2(double &h, double &c, double v)
     h = t;
 }
 
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+// Variant on v_mfma_f32_32x32x16_bf16 with 4 waves of 128 x 128 and LDS
+// filled by DMA (global_load_lds_dwordx4), no staging registers.
+//
+// * Workgroup: 4 waves (2 x 2) on a 256 x 256 tile; wave (wr, wc) owns H
+//   rows wr*128.. and columns wc*128.. as 4 x 4 blocks of 32 x 32 (256 FP32
+//   accumulators per lane).  The MFMA is fed the column side as its A
+//   operand and the row side as its B operand, so its result D(j, i) =
+//   H(i, j) puts 32 consecutive H rows in the 32 lanes of a half-wave:
+//   column-major H and the split-K partials are written in 128/256-byte runs.
+// * Operand layouts (checked on gfx950): lane l supplies row/column l % 32
+//   and contraction indices 8 (l / 32) .. +7; result element e of lane l is
+//   D(8 (e / 4) + 4 (l / 32) + e % 4, l % 32).
+// * LDS: [stage][plane Ah, Al, Bh, Bl][256 rows x 4 chunks of 16 B], the
+//   chunk swizzle of k_syrk_x3.  A DMA instruction writes 64 consecutive
+//   16-byte positions (lane l at base + 16 l), so each lane loads the global
+//   chunk that belongs at its position.  Stage s^1 is filled for step kb+1
+//   while step kb is computed from stage s; one wait + barrier per step.
+// * Per 16-point substep u a lane reads chunk 2u + l/32 of its rows; the
+//   weight signs are applied to the column-side operand by a sign-bit XOR.
+template <int TERMS>
+__global__ void __launch_bounds__(256, 1)
+    k_syrk_x3m(const uint4 *__restrict__ Xh, const uint4 *__restrict__ Xl,
+               const unsigned *__restrict__ sgn, int64_t rows_pad, const int4 *__restrict__ work,
+               int ntiles, int kb_per_split, int nkb, int64_t n, double *__restrict__ H,
+               double *__restrict__ Hc, float *__restrict__ part)
+{
+    __shared__ uint4 lds[2][4][kFT * 4];
+    const int4 wk = work[blockIdx.x];
+    const int split = wk.z, tile = wk.w;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int l32 = lane & 31, kh = lane >> 5;
+    const int kb0 = split * kb_per_split;
+    int kb1 = kb0 + kb_per_split;
+    if (kb1 > nkb) kb1 = nkb;
+    const int64_t rowA = (int64_t)wk.x * kFT, rowB = (int64_t)wk.y * kFT;
+    const size_t planeStride = (size_t)rows_pad * 4;  // uint4 per 32-point chunk plane
+
+    // DMA work of this wave: instructions t = 0 .. NI-1, global instruction
+    // u = wave * NI + t; plane slot (u / 16) of the needed planes, positions
+    // 64 (u % 16) + lane
+    constexpr int NPL = TERMS == 3 ? 4 : 2;
+    constexpr int NI = NPL * 16 / 4;
+    int dma_off[NI];   // uint4 offset within a plane chunk-block, per instruction
+#pragma unroll
+    for (int t = 0; t < NI; ++t) {
+        const int u = wave * NI + t, ip = u & 15;
+        const int p = 64 * ip + lane, r = p >> 2, pc = p & 3;
+        dma_off[t] = r * 4 + (pc ^ ((r >> 2) & 3));
+    }
+    auto dma = [&](int kb, int sg) {
+#pragma unroll
+        for (int t = 0; t < NI; ++t) {
+            const int u = wave * NI + t, ip = u & 15, pl = u >> 4;
+            const int slot = TERMS == 3 ? pl : 2 * pl;        // Ah, Al, Bh, Bl or Ah, Bh
+            const uint4 *src = (slot & 1) ? Xl : Xh;
+            const int64_t row0 = slot >= 2 ? rowB : rowA;
+            const uint4 *g = src + (size_t)kb * planeStride + (size_t)row0 * 4 + dma_off[t];
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(g),
+                                             (__attribute__((address_space(3))) void *)(&lds[sg][slot][64 * ip]),
+                                             16, 0, 0);
+        }
+    };
+
+    f16v acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
+
+    if (kb0 < kb1) dma(kb0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    int s = 0;
+    unsigned sw_next = kb0 < kb1 ? sgn[kb0] : 0u;
+    for (int kb = kb0; kb < kb1; ++kb) {
+        if (kb + 1 < kb1) dma(kb + 1, s ^ 1);
+        const unsigned sw = sw_next;
+        if (kb + 1 < kb1) sw_next = sgn[kb + 1];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            // column side (MFMA A operand, signed) and row side (B operand)
+            const int c = 2 * u + kh;
+            const unsigned bits = (sw >> (16 * u + 8 * kh)) & 0xffu;
+            uint4 fl;
+            fl.x = ((bits & 1u) << 15) | (((bits >> 1) & 1u) << 31);
+            fl.y = (((bits >> 2) & 1u) << 15) | (((bits >> 3) & 1u) << 31);
+            fl.z = (((bits >> 4) & 1u) << 15) | (((bits >> 5) & 1u) << 31);
+            fl.w = (((bits >> 6) & 1u) << 15) | (((bits >> 7) & 1u) << 31);
+            bf16x8 ch[4], cl[4], rh[4], rl[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int r = wc * 128 + 32 * b + l32;
+                const int p = r * 4 + (c ^ ((r >> 2) & 3));
+                uint4 v = lds[s][2][p];
+                v.x ^= fl.x; v.y ^= fl.y; v.z ^= fl.z; v.w ^= fl.w;
+                ch[b] = __builtin_bit_cast(bf16x8, v);
+                if (TERMS == 3) {
+                    uint4 vl = lds[s][3][p];
+                    vl.x ^= fl.x; vl.y ^= fl.y; vl.z ^= fl.z; vl.w ^= fl.w;
+                    cl[b] = __builtin_bit_cast(bf16x8, vl);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int r = wr * 128 + 32 * a + l32;
+                const int p = r * 4 + (c ^ ((r >> 2) & 3));
+                rh[a] = __builtin_bit_cast(bf16x8, lds[s][0][p]);
+                if (TERMS == 3) rl[a] = __builtin_bit_cast(bf16x8, lds[s][1][p]);
+            }
+#pragma unroll
+            for (int term = 0; term < TERMS; ++term)
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const bf16x8 x = term == 2 ? cl[b] : ch[b];
+                        const bf16x8 y = term == 1 ? rl[a] : rh[a];
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc[a][b], 0, 0, 0);
+                    }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        s ^= 1;
+    }
+
+    // D(j, i) of block (a, b): element e of lane l has j = 8 (e/4) + 4 kh + e%4
+    // (column within the block), i = l32 (row within the block).  One block
+    // at a time (sched_barrier): read out together, the 256 accumulators
+    // would not fit the VGPRs next to the addresses.
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int tr = wr * 128 + 32 * a + l32;
+                const int tc = wc * 128 + 32 * b + 8 * (e >> 2) + 4 * kh + (e & 3);
+                const int64_t row = rowA + tr, col = rowB + tc;
+                const float v = acc[a][b][e];
+                if (part) {
+                    part[((size_t)split * ntiles + tile) * (kFT * kFT) + (size_t)tc * kFT + tr] = v;
+                } else if (row <= col && col < n) {
+                    const size_t idx = (size_t)col * n + row;
+                    double h = H[idx], cc = Hc[idx];
+                    kahan_add2(h, cc, (double)v);
+                    H[idx] = h;
+                    Hc[idx] = cc;
+                }
+            }
+        }
+}
+
+// split-K partials of k_syrk_x3m: [split][tile][column][row] (column-major
+// within the tile); same fixed summation order as k_syrk_x3_fold
+__global__ void __launch_bounds__(256)
+    k_syrk_x3m_fold(const float *__restrict__ part, const int2 *__restrict__ tiles, int ntiles,
+                    int nsplit, int64_t n, double *__restrict__ H, double *__restrict__ Hc)
+{
+    const int tile = blockIdx.x;
+    const int2 t = tiles[tile];
+    const int e = blockIdx.y * 256 + threadIdx.x;
+    const int tc = e / kFT, tr = e % kFT;
+    const int64_t row = (int64_t)t.x * kFT + tr, col = (int64_t)t.y * kFT + tc;
+    if (row > col || col >= n) return;
+    const size_t stride = (size_t)ntiles * (kFT * kFT);
+    const float *p = part + (size_t)tile * (kFT * kFT) + e;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int sp = 0;
+    for (; sp + 4 <= nsplit; sp += 4) {
+        s0 += (double)p[(size_t)sp * stride];
+        s1 += (double)p[(size_t)(sp + 1) * stride];
+        s2 += (double)p[(size_t)(sp + 2) * stride];
+        s3 += (double)p[(size_t)(sp + 3) * stride];
+    }
+    for (; sp < nsplit; ++sp) s0 += (double)p[(size_t)sp * stride];
+    const double sum = (s0 + s1) + (s2 + s3);
+    const size_t idx = (size_t)col * n + row;
+    double h = H[idx], c = Hc[idx];
+    kahan_add2(h, c, sum);
+    H[idx] = h;
+    Hc[idx] = c;
+}
+
 // TERMS = 3: hi.hi + hi.lo + lo.hi; TERMS = 1: hi.hi only (relative error
 // ~2^-8 per product; used for the surrogate term of the control variate)
 template <int TERMS>
@@ -468,21 +657,31 @@ hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
     const int4 *work;
     int nwork;
     if ((e = work_order(ctx, n, nsplit, &work, &nwork)) != hipSuccess) return e;
+    const uint4 *xh = reinterpret_cast<const uint4 *>(Xh), *xl = reinterpret_cast<const uint4 *>(Xl);
+    const bool m32 = ctx->x3_variant == 2;
     hipEvent_t ev;
     timer_begin(ctx, "syrk_mma", &ev);
-    if (terms == 1)
-        hipLaunchKernelGGL(k_syrk_x3<1>, dim3((unsigned)nwork), dim3(512), 0, ctx->stream,
-                           reinterpret_cast<const uint4 *>(Xh), reinterpret_cast<const uint4 *>(Xl), sgn,
-                           rows_pad, work, ntiles, kbs, nkb, n, H, Hc, part);
+    if (m32 && terms == 1)
+        hipLaunchKernelGGL(k_syrk_x3m<1>, dim3((unsigned)nwork), dim3(256), 0, ctx->stream, xh, xl, sgn, rows_pad,
+                           work, ntiles, kbs, nkb, n, H, Hc, part);
+    else if (m32)
+        hipLaunchKernelGGL(k_syrk_x3m<3>, dim3((unsigned)nwork), dim3(256), 0, ctx->stream, xh, xl, sgn, rows_pad,
+                           work, ntiles, kbs, nkb, n, H, Hc, part);
+    else if (terms == 1)
+        hipLaunchKernelGGL(k_syrk_x3<1>, dim3((unsigned)nwork), dim3(512), 0, ctx->stream, xh, xl, sgn, rows_pad,
+                           work, ntiles, kbs, nkb, n, H, Hc, part);
     else
-        hipLaunchKernelGGL(k_syrk_x3<3>, dim3((unsigned)nwork), dim3(512), 0, ctx->stream,
-                           reinterpret_cast<const uint4 *>(Xh), reinterpret_cast<const uint4 *>(Xl), sgn,
-                           rows_pad, work, ntiles, kbs, nkb, n, H, Hc, part);
+        hipLaunchKernelGGL(k_syrk_x3<3>, dim3((unsigned)nwork), dim3(512), 0, ctx->stream, xh, xl, sgn, rows_pad,
+                           work, ntiles, kbs, nkb, n, H, Hc, part);
     timer_end(ctx, "syrk_mma", ev);
     e = hipGetLastError();
     if (e != hipSuccess || !part) return e;
-    hipLaunchKernelGGL(k_syrk_x3_fold, dim3((unsigned)ntiles, kFT * kFT / 256), dim3(256), 0, ctx->stream, part,
-                       tiles, ntiles, nsplit, n, H, Hc);
+    if (m32)
+        hipLaunchKernelGGL(k_syrk_x3m_fold, dim3((unsigned)ntiles, kFT * kFT / 256), dim3(256), 0, ctx->stream,
+                           part, tiles, ntiles, nsplit, n, H, Hc);
+    else
+        hipLaunchKernelGGL(k_syrk_x3_fold, dim3((unsigned)ntiles, kFT * kFT / 256), dim3(256), 0, ctx->stream,
+                           part, tiles, ntiles, nsplit, n, H, Hc);
     return hipGetLastError();
 }
 
