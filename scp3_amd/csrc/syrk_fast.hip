@@ -34,6 +34,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kFT = 256;  // workgroup tile
+constexpr int kMaxSignWords = 4096;  // 32-point sign words per split (131072 points)
 
 __device__ __forceinline__ unsigned short bf16_rne_u(float f)
 {
@@ -140,6 +141,9 @@ __global__ void __launch_bounds__(256, 1)
                double *__restrict__ Hc, float *__restrict__ part)
 {
     __shared__ uint4 lds[2][4][kFT * 4];
+    // the sign words of the workgroup's K range, read once: a scalar load
+    // per step would make every LDS-read wait (lgkmcnt) cover it too
+    __shared__ unsigned sgl[kMaxSignWords];
     const int4 wk = work[blockIdx.x];
     const int split = wk.z, tile = wk.w;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -186,14 +190,16 @@ __global__ void __launch_bounds__(256, 1)
             for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
 
     if (kb0 < kb1) dma(kb0, 0);
+    for (int i = tid; i < kb1 - kb0; i += 256) sgl[i] = sgn[kb0 + i];
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     int s = 0;
-    unsigned sw_next = kb0 < kb1 ? sgn[kb0] : 0u;
     for (int kb = kb0; kb < kb1; ++kb) {
-        if (kb + 1 < kb1) dma(kb + 1, s ^ 1);
-        const unsigned sw = sw_next;
-        if (kb + 1 < kb1) sw_next = sgn[kb + 1];
+        const unsigned sw = sgl[kb - kb0];
+        // both substeps' operands first: the compiler makes LDS reads wait
+        // for outstanding LDS DMA, so the DMA of step kb+1 is issued after
+        // the last read of step kb and has the step's MFMAs to complete
+        bf16x8 ch[2][4], cl[2][4], rh[2][4], rl[2][4];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             // column side (MFMA A operand, signed) and row side (B operand)
@@ -204,38 +210,44 @@ __global__ void __launch_bounds__(256, 1)
             fl.y = (((bits >> 2) & 1u) << 15) | (((bits >> 3) & 1u) << 31);
             fl.z = (((bits >> 4) & 1u) << 15) | (((bits >> 5) & 1u) << 31);
             fl.w = (((bits >> 6) & 1u) << 15) | (((bits >> 7) & 1u) << 31);
-            bf16x8 ch[4], cl[4], rh[4], rl[4];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int r = wc * 128 + 32 * b + l32;
                 const int p = r * 4 + (c ^ ((r >> 2) & 3));
                 uint4 v = lds[s][2][p];
                 v.x ^= fl.x; v.y ^= fl.y; v.z ^= fl.z; v.w ^= fl.w;
-                ch[b] = __builtin_bit_cast(bf16x8, v);
+                ch[u][b] = __builtin_bit_cast(bf16x8, v);
                 if (TERMS == 3) {
                     uint4 vl = lds[s][3][p];
                     vl.x ^= fl.x; vl.y ^= fl.y; vl.z ^= fl.z; vl.w ^= fl.w;
-                    cl[b] = __builtin_bit_cast(bf16x8, vl);
+                    cl[u][b] = __builtin_bit_cast(bf16x8, vl);
                 }
             }
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
                 const int r = wr * 128 + 32 * a + l32;
                 const int p = r * 4 + (c ^ ((r >> 2) & 3));
-                rh[a] = __builtin_bit_cast(bf16x8, lds[s][0][p]);
-                if (TERMS == 3) rl[a] = __builtin_bit_cast(bf16x8, lds[s][1][p]);
+                rh[u][a] = __builtin_bit_cast(bf16x8, lds[s][0][p]);
+                if (TERMS == 3) rl[u][a] = __builtin_bit_cast(bf16x8, lds[s][1][p]);
             }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (kb + 1 < kb1) dma(kb + 1, s ^ 1);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
 #pragma unroll
             for (int term = 0; term < TERMS; ++term)
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
                     for (int b = 0; b < 4; ++b) {
-                        const bf16x8 x = term == 2 ? cl[b] : ch[b];
-                        const bf16x8 y = term == 1 ? rl[a] : rh[a];
+                        const bf16x8 x = term == 2 ? cl[u][b] : ch[u][b];
+                        const bf16x8 y = term == 1 ? rl[u][a] : rh[u][a];
                         acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc[a][b], 0, 0, 0);
                     }
-        }
+        // the wait for the DMA and the barrier stay behind the MFMAs
+        __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         s ^= 1;
@@ -641,6 +653,7 @@ hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
     while (ntiles * nsplit < 512 && nkb / (nsplit * 2) >= 8) nsplit *= 2;
     const int kbs = (nkb + nsplit - 1) / nsplit;
     nsplit = (nkb + kbs - 1) / kbs;
+    if (kbs > kMaxSignWords) return hipErrorInvalidValue;  // tile above 131072 points
     float *part = nullptr;
     if (nsplit > 1) {
         const size_t need = (size_t)nsplit * ntiles * kFT * kFT * sizeof(float);
