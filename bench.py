@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--cpu-chunk", type=int, default=8192, help="points per CPU baseline chunk")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cv-terms", type=int, default=1)
+    ap.add_argument("--mixed-kernel", type=int, default=None, help="SYRK kernel variant (scp3g_set_mixed_kernel)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     return ap.parse_args()
@@ -79,6 +80,8 @@ def setup(c, sysd, pr, mode):
     c.set_pes("ttm3f" if "ttm3f" in pr["config"] else "qtip4pf")
     c.set_precision(mode)
     c.set_tile(pr["tile"])
+    if pr.get("mixed_kernel") is not None:
+        c.set_mixed_kernel(pr["mixed_kernel"])
 
 
 def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
@@ -190,6 +193,7 @@ def main():
         pr["points"] = a.points
     if a.forms:
         pr["forms"] = a.forms
+    pr["mixed_kernel"] = a.mixed_kernel
     sysd = system.prepare(system.input_path(pr["config"]), nmodes2=pr["n2"], nmodes3=pr["n3"])
     sysd["name"] = pr["config"]
     P = 1 << pr["points"]

@@ -188,8 +188,9 @@ class Context:
     def set_tile(self, tile):
         self.check(self.L.scp3g_set_tile(self._h, tile))
 
-    def set_mixed_kernel(self, fast):
-        self.check(self.L.scp3g_set_mixed_kernel(self._h, 1 if fast else 0))
+    def set_mixed_kernel(self, kernel):
+        """0: 64 x 64 direct-load SYRKs; 1, 2: LDS-tiled SYRKs (see scp3g.h)"""
+        self.check(self.L.scp3g_set_mixed_kernel(self._h, int(kernel)))
 
     # whole path ---------------------------------------------------------
     def reset(self):

@@ -64,7 +64,7 @@ struct scp3g_ctx {
     bool mixed_fast = true;
     // signed split-BF16 SYRK variant: 1 = 8 waves of 128 x 64 on 16x16x32
     // MFMA, 2 = 4 waves of 128 x 128 on 32x32x16 MFMA with LDS DMA
-    int x3_variant = 2;
+    int x3_variant = 1;
     // control variate, mixed form: the surrogate term with hi.hi products only
     bool cv_one_term = false;
 
