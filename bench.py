@@ -134,6 +134,21 @@ def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
                 points=npts, train_s=train_s)
 
 
+def pmc_traffic(kernel, config, n, points_per_launch):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    FETCH_SIZE / WRITE_SIZE summary (rocprofv3 --pmc runs, scaled by a
+    plain-copy calibration), or None when that workload was not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic_syrk_x3_hexamer.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 6 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and f[2] == config \
+                and int(f[3]) == n and int(f[4]) == int(round(points_per_launch)):
+            return float(f[5])
+    return None
+
+
 def cpu_baseline(sysd, skip, chunk, target_s, max_points):
     """FP64 oracle path on the host over a bounded prefix of the same points:
     chunks of `chunk` points until `target_s` seconds or `max_points`."""
@@ -235,7 +250,9 @@ def main():
                                % (a.preset, pr["config"], n, pr["points"], round(pts_per_launch), pr["levels"]),
                    "basis_states": n, "points": P, "form": main_form},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": pmc_traffic("k_syrk_x3<3>", pr["config"], n, pts_per_launch)
+                     if main_form == "mixed" else None,
                      "kernel": "%s (%s form)" % ("k_syrk_f64_lds" if main_form == "fp64" else "k_syrk_x3", main_form),
                      "avg_launch_ms": avg_s * 1e3,
                      "flops_per_launch": flops},
