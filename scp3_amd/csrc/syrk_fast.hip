@@ -59,48 +59,83 @@ __global__ void __launch_bounds__(256)
 }
 
 // x = sqrt|w| * bra in FP32, split into hi/lo BF16 planes in the k-blocked
-// layout.  A block holds PB points (64, or fewer when many modes are
-// selected) and 256/PB row groups, as in k_basis.
+// layout.  A block holds 64 points (two 32-point chunk blocks) and a range of
+// rows.  Phase 1 puts the Hermite values h1..h3 of the selected modes and
+// sqrt|w| of its points in LDS.  Phase 2: thread t takes chunk block t / 128,
+// points 8 (t % 4) .. +7 of it, and row (t % 128) / 4 of every 32-row pass;
+// it forms the 8 products of its row and stores them as
+// one 16-byte run per plane, so a wave writes 16 consecutive rows x 64 B =
+// 1 KiB of contiguous memory per plane.  Product order (f0 f1) f2 times
+// sqrt|w|, as in get_bra (ScaledMatrixElements.cpp:870-930).
 __global__ void __launch_bounds__(256)
     k_basis_split(const double *__restrict__ y, int64_t ldy, const int *__restrict__ modes, int nsel,
                   const RowDesc *__restrict__ desc, int64_t n, int64_t rows_pad, int rows_per_block,
                   int64_t npts, int64_t K, const double *__restrict__ w,
-                  unsigned short *__restrict__ Xh, unsigned short *__restrict__ Xl, int pb_log2)
+                  uint4 *__restrict__ Xh, uint4 *__restrict__ Xl)
 {
-    extern __shared__ float hsf[];  // [3][nsel][PB]
-    const int PB = 1 << pb_log2, ng = 256 >> pb_log2;
-    const int lane = threadIdx.x & (PB - 1), grp = threadIdx.x >> pb_log2;
-    const int64_t i = (int64_t)blockIdx.x * PB + lane;
-    const bool live = i < npts;
-    for (int m = grp; m < nsel; m += ng) {
-        const float q = live ? (float)y[(size_t)modes[m] * ldy + i] : 0.f;
-        hsf[(0 * nsel + m) * PB + lane] = 1.41421356f * q;
-        hsf[(1 * nsel + m) * PB + lane] = (2.f * q * q - 1.f) * 0.70710678f;
-        hsf[(2 * nsel + m) * PB + lane] = (2.f * q * q - 3.f) * q * 0.57735027f;
+    constexpr int PB = 64;
+    extern __shared__ float hsf[];  // [3][nsel][PB], then sqrt|w| [PB]
+    float *sws = hsf + (size_t)3 * nsel * PB;
+    const int tid = threadIdx.x;
+    {
+        const int lane = tid & (PB - 1), grp = tid >> 6;
+        const int64_t i = (int64_t)blockIdx.x * PB + lane;
+        const bool live = i < npts;
+        for (int m = grp; m < nsel; m += 4) {
+            const float q = live ? (float)y[(size_t)modes[m] * ldy + i] : 0.f;
+            hsf[(0 * nsel + m) * PB + lane] = 1.41421356f * q;
+            hsf[(1 * nsel + m) * PB + lane] = (2.f * q * q - 1.f) * 0.70710678f;
+            hsf[(2 * nsel + m) * PB + lane] = (2.f * q * q - 3.f) * q * 0.57735027f;
+        }
+        if (grp == 0) sws[lane] = live ? (float)sqrt(fabs(w[i])) : 0.f;
     }
     __syncthreads();
-    const float sw = live ? (float)sqrt(fabs(w[i])) : 0.f;
+    const int half = tid >> 7, idx = tid & 127, rl = idx >> 2, pg = idx & 3;
+    const int p0 = half * 32 + pg * 8;  // first of this thread's 8 points in the block
+    const int64_t kb = (int64_t)blockIdx.x * 2 + half;
+    if (kb * 32 >= K) return;
+    float sw[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sw[j] = sws[p0 + j];
     const int64_t a0 = (int64_t)blockIdx.y * rows_per_block;
     int64_t a1 = a0 + rows_per_block;
     if (a1 > rows_pad) a1 = rows_pad;
-    if (i >= K) return;
-    const size_t chunk = (size_t)(i >> 5) * rows_pad * 32 + (i & 31);
-    for (int64_t a = a0 + grp; a < a1; a += ng) {
-        float v = 0.f;
-        if (a < n && live) {
+    uint4 *ph = Xh + (size_t)kb * rows_pad * 4 + pg;
+    uint4 *pl = Xl + (size_t)kb * rows_pad * 4 + pg;
+    for (int64_t a = a0 + rl; a < a1; a += 32) {
+        float v[8];
+        if (a < n) {
             const RowDesc d = desc[a];
-            float f[3];
+            float f[3][8];
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
                 const int mode = d.f[t] & 0xffff, deg = d.f[t] >> 16;
-                f[t] = deg ? hsf[((deg - 1) * nsel + mode) * PB + lane] : 1.f;
+                if (deg) {
+                    const float4 *src = reinterpret_cast<const float4 *>(hsf + ((deg - 1) * nsel + mode) * PB + p0);
+                    const float4 u0 = src[0], u1 = src[1];
+                    f[t][0] = u0.x; f[t][1] = u0.y; f[t][2] = u0.z; f[t][3] = u0.w;
+                    f[t][4] = u1.x; f[t][5] = u1.y; f[t][6] = u1.z; f[t][7] = u1.w;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) f[t][j] = 1.f;
+                }
             }
-            v = sw * ((f[0] * f[1]) * f[2]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = sw[j] * ((f[0][j] * f[1][j]) * f[2][j]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = 0.f;
         }
-        const unsigned short h = bf16_rne_u(v);
-        const unsigned short l = bf16_rne_u(v - __uint_as_float((unsigned)h << 16));
-        Xh[chunk + (size_t)a * 32] = h;
-        Xl[chunk + (size_t)a * 32] = l;
+        unsigned hh[8], ll[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            hh[j] = bf16_rne_u(v[j]);
+            ll[j] = bf16_rne_u(v[j] - __uint_as_float(hh[j] << 16));
+        }
+        ph[a * 4] = make_uint4(hh[0] | (hh[1] << 16), hh[2] | (hh[3] << 16), hh[4] | (hh[5] << 16),
+                               hh[6] | (hh[7] << 16));
+        pl[a * 4] = make_uint4(ll[0] | (ll[1] << 16), ll[2] | (ll[3] << 16), ll[4] | (ll[5] << 16),
+                               ll[6] | (ll[7] << 16));
     }
 }
 
@@ -568,18 +603,22 @@ hipError_t launch_basis_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int6
 {
     const int64_t n = ctx->nbasis, rows_pad = fast_rows_pad(n);
     const int nsel = (int)ctx->modes.size();
-    int pb_log2 = 6;
-    while (pb_log2 > 3 && (size_t)3 * nsel * (1u << pb_log2) * sizeof(float) > 64 * 1024) --pb_log2;
-    const size_t smem = (size_t)3 * nsel * (1u << pb_log2) * sizeof(float);
-    if (smem > 64 * 1024) return hipErrorInvalidValue;
+    // 64 points per block: Hermite values of every selected mode and the
+    // point weights (up to 134 KiB for 174 modes; the CU has 160 KiB)
+    const size_t smem = ((size_t)3 * nsel * 64 + 64) * sizeof(float);
+    if (smem > 160 * 1024 || K % 64) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_basis_split),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sign_words, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, ctx->stream, w, npts, K,
                        sgn);
-    const int64_t pblocks = K >> pb_log2;
-    int rpb = 256;
-    while (rpb > 16 && pblocks * ((rows_pad + rpb - 1) / rpb) < 2048) rpb /= 2;
+    const int64_t pblocks = K / 64;
+    int64_t rpb = (rows_pad + 31) / 32 * 32;
+    while (rpb > 256 && pblocks * ((rows_pad + rpb - 1) / rpb) < 2048) rpb = (rpb / 2 + 31) / 32 * 32;
     dim3 grid((unsigned)pblocks, (unsigned)((rows_pad + rpb - 1) / rpb));
     hipLaunchKernelGGL(k_basis_split, grid, dim3(256), smem, ctx->stream, y, ldy, ctx->d_modes, nsel,
-                       ctx->d_desc, n, rows_pad, rpb, npts, K, w, Xh, Xl, pb_log2);
+                       ctx->d_desc, n, rows_pad, (int)rpb, npts, K, w, reinterpret_cast<uint4 *>(Xh),
+                       reinterpret_cast<uint4 *>(Xl));
     return hipGetLastError();
 }
 
