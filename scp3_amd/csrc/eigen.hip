@@ -165,86 +165,92 @@ __global__ void __launch_bounds__(256)
     Y[(size_t)j * ldy + r] = s;
 }
 
-// C(p x q) = P^T Q for tall P (n x p, ld ldp) and Q (n x q, ld ldq).  A
-// workgroup owns a 4 x 8 block of C and reduces over n (strided by 256,
-// then across lanes and waves in a fixed order).
-__global__ void __launch_bounds__(256)
+// C(p x q) = P^T Q for tall P (n x p, ld ldp) and Q (n x q, ld ldq).  One
+// wave owns a 4 x 4 block of C: lanes stride over n, then a butterfly of
+// shuffles reduces the 16 sums (fixed order, no LDS, no barrier; the
+// Davidson sizes make this a latency problem, not a bandwidth one).
+__global__ void __launch_bounds__(64)
     k_gemm_tn(const double *__restrict__ Pm, int64_t ldp, int p, const double *__restrict__ Q,
               int64_t ldq, int q, int64_t n, double *__restrict__ C, int ldc)
 {
-    __shared__ double red[4][32];
-    const int i0 = blockIdx.x * 4, j0 = blockIdx.y * 8;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double acc[4][8];
+    const int i0 = blockIdx.x * 4, j0 = blockIdx.y * 4;
+    const int lane = threadIdx.x;
+    const double *pc[4], *qc[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        pc[a] = Pm + (size_t)(i0 + a < p ? i0 + a : i0) * ldp;
+        qc[a] = Q + (size_t)(j0 + a < q ? j0 + a : j0) * ldq;
+    }
+    double acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc[a][c] = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += 256) {
-        double pv[4], qv[8];
+        for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+#pragma unroll 2
+    for (int64_t k = lane; k < n; k += 64) {
+        double pv[4], qv[4];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) pv[a] = (i0 + a < p) ? Pm[(size_t)(i0 + a) * ldp + k] : 0.0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) qv[c] = (j0 + c < q) ? Q[(size_t)(j0 + c) * ldq + k] : 0.0;
+        for (int a = 0; a < 4; ++a) {
+            pv[a] = pc[a][k];
+            qv[a] = qc[a][k];
+        }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int c = 0; c < 8; ++c) acc[a][c] = fma(pv[a], qv[c], acc[a][c]);
+            for (int c = 0; c < 4; ++c) acc[a][c] = fma(pv[a], qv[c], acc[a][c]);
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
+        for (int c = 0; c < 4; ++c) {
             double v = acc[a][c];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
             acc[a][c] = v;
         }
-    if (lane == 0)
+    if (lane < 16) {
+        const int a = lane >> 2, c = lane & 3;
+        double v = 0.0;
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int aa = 0; aa < 4; ++aa)
 #pragma unroll
-            for (int c = 0; c < 8; ++c) red[wave][a * 8 + c] = acc[a][c];
-    __syncthreads();
-    if (threadIdx.x < 32) {
-        const int a = threadIdx.x >> 3, c = threadIdx.x & 7;
-        const double v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+            for (int cc = 0; cc < 4; ++cc)
+                if (aa == a && cc == c) v = acc[aa][cc];
         if (i0 + a < p && j0 + c < q) C[(size_t)(j0 + c) * ldc + i0 + a] = v;
     }
 }
 
 // Y(n x q) = beta*Y + alpha * P(n x p) S(p x q), q <= kMaxB; S column-major
-// with ld lds, staged through LDS 64 rows at a time; P is read once.
-__global__ void __launch_bounds__(256)
+// with ld lds.  One wave owns 64 rows and 8 columns: the 8 columns of S are
+// staged in LDS once, each lane streams its row of P (one load per k).
+__global__ void __launch_bounds__(64)
     k_gemm_nn(const double *__restrict__ Pm, int64_t ldp, int p, const double *__restrict__ S,
               int lds, int q, double alpha, double beta, double *__restrict__ Y, int64_t ldy,
               int64_t n)
 {
-    __shared__ double ss[64][kMaxB];
-    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    double acc[kMaxB];
-#pragma unroll
-    for (int j = 0; j < kMaxB; ++j) acc[j] = 0.0;
-    for (int k0 = 0; k0 < p; k0 += 64) {
-        const int kn = p - k0 < 64 ? p - k0 : 64;
-        __syncthreads();
-        for (int e = threadIdx.x; e < 64 * kMaxB; e += 256) {
-            const int kk = e & 63, j = e >> 6;
-            ss[kk][j] = (kk < kn && j < q) ? S[(size_t)j * lds + k0 + kk] : 0.0;
-        }
-        __syncthreads();
-        if (r < n)
-            for (int kk = 0; kk < kn; ++kk) {
-                const double pv = Pm[(size_t)(k0 + kk) * ldp + r];
-#pragma unroll
-                for (int j = 0; j < kMaxB; ++j) acc[j] = fma(pv, ss[kk][j], acc[j]);
-            }
+    constexpr int kMaxP = 256;
+    __shared__ double ss[kMaxP][8];
+    const int64_t r = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int j0 = blockIdx.y * 8;
+    for (int e = threadIdx.x; e < p * 8; e += 64) {
+        const int k = e >> 3, j = e & 7;
+        ss[k][j] = (j0 + j < q) ? S[(size_t)(j0 + j) * lds + k] : 0.0;
     }
+    __syncthreads();
     if (r >= n) return;
+    double acc[8];
 #pragma unroll
-    for (int j = 0; j < kMaxB; ++j)
-        if (j < q) {
-            double *yp = Y + (size_t)j * ldy + r;
+    for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+#pragma unroll 4
+    for (int k = 0; k < p; ++k) {
+        const double pv = Pm[(size_t)k * ldp + r];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] = fma(pv, ss[k][j], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (j0 + j < q) {
+            double *yp = Y + (size_t)(j0 + j) * ldy + r;
             *yp = (beta == 0.0 ? 0.0 : beta * *yp) + alpha * acc[j];
         }
 }
@@ -346,14 +352,16 @@ struct Dav {
     }
     hipError_t tn(const double *Pm, int p, const double *Q, int q, double *C, int ldc)
     {
-        hipLaunchKernelGGL(k_gemm_tn, dim3((unsigned)((p + 3) / 4), (unsigned)((q + 7) / 8)), dim3(256), 0,
+        hipLaunchKernelGGL(k_gemm_tn, dim3((unsigned)((p + 3) / 4), (unsigned)((q + 3) / 4)), dim3(64), 0,
                            ctx->stream, Pm, n, p, Q, n, q, n, C, ldc);
         return hipGetLastError();
     }
     hipError_t nn(const double *Pm, int p, const double *Sm, int lds, int q, double alpha,
                   double beta, double *Y)
     {
-        hipLaunchKernelGGL(k_gemm_nn, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, Pm, n, p,
+        if (p > 256) return hipErrorInvalidValue;  // k_gemm_nn stages at most 256 rows of S
+        hipLaunchKernelGGL(k_gemm_nn, dim3((unsigned)((n + 63) / 64), (unsigned)((q + 7) / 8)), dim3(64), 0,
+                           ctx->stream, Pm, n, p,
                            Sm, lds, q, alpha, beta, Y, n, n);
         return hipGetLastError();
     }
@@ -562,21 +570,26 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
             DV_CHECK(dv.tn(dv.V, m, dv.T, b, dv.G, m), "VtT");
             DV_CHECK(dv.nn(dv.V, m, dv.G, m, b, -1.0, 1.0, dv.T), "T-=VC");
         }
-        // CholQR2 within T (drop to fewer columns if rank deficient)
+        // CholQR2 within T (drop to fewer columns if rank deficient); the
+        // second pass writes the new block straight into V's next columns
         int nb = b;
-        for (int pass = 0; pass < 2; ++pass) {
-            DV_CHECK(dv.tn(dv.T, nb, dv.T, nb, dv.G, nb), "TtT");
-            std::vector<double> U((size_t)nb * nb);
-            DV_CHECK(hipMemcpyAsync(U.data(), dv.G, U.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "TtT");
-            DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
-            if (!cholesky_upper(nb, U)) {
-                nb = 0;
-                break;
+        {
+            const double *src = dv.T;
+            for (int pass = 0; pass < 2; ++pass) {
+                double *dst = pass == 0 ? dv.AT : dv.V + (size_t)m * n;
+                DV_CHECK(dv.tn(src, nb, src, nb, dv.G, nb), "TtT");
+                std::vector<double> U((size_t)nb * nb);
+                DV_CHECK(hipMemcpyAsync(U.data(), dv.G, U.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "TtT");
+                DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
+                if (!cholesky_upper(nb, U)) {
+                    nb = 0;
+                    break;
+                }
+                std::vector<double> Ui = upper_inverse(nb, U);
+                DV_CHECK(hipMemcpyAsync(dv.S, Ui.data(), Ui.size() * 8, hipMemcpyHostToDevice, ctx->stream), "Ui");
+                DV_CHECK(dv.nn(src, nb, dv.S, nb, nb, 1.0, 0.0, dst), "T U^-1");
+                src = dst;
             }
-            std::vector<double> Ui = upper_inverse(nb, U);
-            DV_CHECK(hipMemcpyAsync(dv.S, Ui.data(), Ui.size() * 8, hipMemcpyHostToDevice, ctx->stream), "Ui");
-            DV_CHECK(dv.nn(dv.T, nb, dv.S, nb, nb, 1.0, 0.0, dv.AT), "T U^-1");
-            DV_CHECK(hipMemcpyAsync(dv.T, dv.AT, nn_ * nb * 8, hipMemcpyDeviceToDevice, ctx->stream), "T");
         }
         if (nb == 0) {
             // correction collapsed: fall back to the (orthogonalised) residuals
@@ -597,12 +610,8 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
             }
             std::vector<double> Ui = upper_inverse(nb, U);
             DV_CHECK(hipMemcpyAsync(dv.S, Ui.data(), Ui.size() * 8, hipMemcpyHostToDevice, ctx->stream), "Ui");
-            DV_CHECK(dv.nn(dv.T, nb, dv.S, nb, nb, 1.0, 0.0, dv.AT), "R U^-1");
-            DV_CHECK(hipMemcpyAsync(dv.T, dv.AT, nn_ * nb * 8, hipMemcpyDeviceToDevice, ctx->stream), "T");
+            DV_CHECK(dv.nn(dv.T, nb, dv.S, nb, nb, 1.0, 0.0, dv.V + (size_t)m * n), "R U^-1");
         }
-        // append
-        DV_CHECK(hipMemcpyAsync(dv.V + (size_t)m * n, dv.T, nn_ * nb * 8, hipMemcpyDeviceToDevice, ctx->stream),
-                 "append V");
         DV_CHECK(dv.symm(dv.V + (size_t)m * n, nb, dv.W + (size_t)m * n), "AT");
         m0 = restarted ? 0 : m;
         m += nb;
