@@ -21,9 +21,11 @@
 // each wave owns 128 x 64 = 8 x 4 MFMA blocks (128 FP32 accumulators per
 // lane), two waves per SIMD.  Per 32-point step the workgroup stages A-hi, A-lo, B-hi, B-lo
 // (64 KiB) in LDS, double buffered (128 KiB).  LDS rows are 64 bytes (four
-// 16-byte chunks); chunk c of row r is stored at position c ^ ((r >> 2) & 3)
-// so that the 16 lanes of an MFMA operand read (rows r..r+15, one chunk)
-// hit 16 distinct 4-bank groups.  The epilogue folds each tile's FP32 sum
+// 16-byte chunks); chunk c of row r is stored at position c ^ ((r >> 1) & 3)
+// so that an MFMA operand read (16 lanes on rows r..r+15, one chunk each) is
+// free of bank conflicts (measured with SQ_LDS_BANK_CONFLICT: the swizzle
+// (r >> 2) & 3 costs 4 conflict cycles per ds_read_b128 for this pattern,
+// (r >> 1) & 3 none).  The epilogue folds each tile's FP32 sum
 // into the FP64 running H with Kahan compensation (Hc), or writes split-K
 // partials.
 #include "scp3g_internal.h"
@@ -386,8 +388,8 @@ __global__ void __launch_bounds__(512, 1)
     // idle buffer) and sched_barriers keep the stores, with their wait for
     // memory, behind the MFMAs.  TERMS = 1 never touches the lo planes.
     const int q0 = tid, q1 = tid + 512;
-    const int p0 = (q0 >> 2) * 4 + ((q0 & 3) ^ ((q0 >> 4) & 3));
-    const int p1 = (q1 >> 2) * 4 + ((q1 & 3) ^ ((q1 >> 4) & 3));
+    const int p0 = (q0 >> 2) * 4 + ((q0 & 3) ^ ((q0 >> 3) & 3));
+    const int p1 = (q1 >> 2) * 4 + ((q1 & 3) ^ ((q1 >> 3) & 3));
     uint4 sh0, sh1, sl0, sl1;
 #define X3_GLOAD(kbv, row0)                                                   \
     do {                                                                      \
@@ -433,7 +435,7 @@ __global__ void __launch_bounds__(512, 1)
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
             const int r = wr * 128 + 16 * a + li;
-            const int p = r * 4 + (lg ^ ((r >> 2) & 3));
+            const int p = r * 4 + (lg ^ ((r >> 1) & 3));
             uint4 vh = lds[s][0][p];
             vh.x ^= fl.x; vh.y ^= fl.y; vh.z ^= fl.z; vh.w ^= fl.w;
             ah[a] = __builtin_bit_cast(bf16x8, vh);
@@ -454,7 +456,7 @@ __global__ void __launch_bounds__(512, 1)
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb) {
                 const int r = wc * 64 + 16 * (2 * half + bb) + li;
-                const int p = r * 4 + (lg ^ ((r >> 2) & 3));
+                const int p = r * 4 + (lg ^ ((r >> 1) & 3));
                 bh[bb] = __builtin_bit_cast(bf16x8, lds[s][2][p]);
                 if (TERMS == 3) bl[bb] = __builtin_bit_cast(bf16x8, lds[s][3][p]);
             }
