@@ -151,23 +151,25 @@ __device__ __forceinline__ void kahan_add2(double &h, double &c, double v)
 
 typedef float f16v __attribute__((ext_vector_type(16)));
 
-// Variant on v_mfma_f32_32x32x16_bf16 with 4 waves of 128 x 128 and LDS
-// filled by DMA (global_load_lds_dwordx4), no staging registers.
+// Variant on v_mfma_f32_32x32x16_bf16 with 4 waves of 128 x 128.
 //
 // * Workgroup: 4 waves (2 x 2) on a 256 x 256 tile; wave (wr, wc) owns H
 //   rows wr*128.. and columns wc*128.. as 4 x 4 blocks of 32 x 32 (256 FP32
-//   accumulators per lane).  The MFMA is fed the column side as its A
-//   operand and the row side as its B operand, so its result D(j, i) =
-//   H(i, j) puts 32 consecutive H rows in the 32 lanes of a half-wave:
-//   column-major H and the split-K partials are written in 128/256-byte runs.
+//   accumulators per lane, in AGPRs).  Per 32-point step a CU reads 128 KiB
+//   of operands from LDS against 192 KiB for the 8-wave 128 x 64 layout.
+//   The MFMA is fed the column side as its A operand and the row side as
+//   its B operand, so its result D(j, i) = H(i, j) puts 32 consecutive H
+//   rows in the 32 lanes of a half-wave: column-major H and the split-K
+//   partials are written in 128/256-byte runs.
 // * Operand layouts (checked on gfx950): lane l supplies row/column l % 32
 //   and contraction indices 8 (l / 32) .. +7; result element e of lane l is
 //   D(8 (e / 4) + 4 (l / 32) + e % 4, l % 32).
-// * LDS: [stage][plane Ah, Al, Bh, Bl][256 rows x 4 chunks of 16 B], the
-//   chunk swizzle of k_syrk_x3.  A DMA instruction writes 64 consecutive
-//   16-byte positions (lane l at base + 16 l), so each lane loads the global
-//   chunk that belongs at its position.  Stage s^1 is filled for step kb+1
-//   while step kb is computed from stage s; one wait + barrier per step.
+// * LDS: [stage][plane Ah, Al, Bh, Bl][256 rows x 4 chunks of 16 B], double
+//   buffered; chunk c of row r at position c ^ ((r >> 2) & 3), free of bank
+//   conflicts for these reads (SQ_LDS_BANK_CONFLICT).  Step kb+1 is loaded
+//   into named registers (16 chunks per thread; an array would be promoted
+//   to LDS scratch) at the start of step kb and stored to the idle stage
+//   after its MFMAs; sched_barriers keep that order.
 // * Per 16-point substep u a lane reads chunk 2u + l/32 of its rows; the
 //   weight signs are applied to the column-side operand by a sign-bit XOR.
 template <int TERMS>
@@ -178,8 +180,7 @@ __global__ void __launch_bounds__(256, 1)
                double *__restrict__ Hc, float *__restrict__ part)
 {
     __shared__ uint4 lds[2][4][kFT * 4];
-    // the sign words of the workgroup's K range, read once: a scalar load
-    // per step would make every LDS-read wait (lgkmcnt) cover it too
+    // the sign words of the workgroup's K range, read once into LDS
     __shared__ unsigned sgl[kMaxSignWords];
     const int4 wk = work[blockIdx.x];
     const int split = wk.z, tile = wk.w;
@@ -192,31 +193,44 @@ __global__ void __launch_bounds__(256, 1)
     const int64_t rowA = (int64_t)wk.x * kFT, rowB = (int64_t)wk.y * kFT;
     const size_t planeStride = (size_t)rows_pad * 4;  // uint4 per 32-point chunk plane
 
-    // DMA work of this wave: instructions t = 0 .. NI-1, global instruction
-    // u = wave * NI + t; plane slot (u / 16) of the needed planes, positions
-    // 64 (u % 16) + lane
-    constexpr int NPL = TERMS == 3 ? 4 : 2;
-    constexpr int NI = NPL * 16 / 4;
-    int dma_off[NI];   // uint4 offset within a plane chunk-block, per instruction
-#pragma unroll
-    for (int t = 0; t < NI; ++t) {
-        const int u = wave * NI + t, ip = u & 15;
-        const int p = 64 * ip + lane, r = p >> 2, pc = p & 3;
-        dma_off[t] = r * 4 + (pc ^ ((r >> 2) & 3));
-    }
-    auto dma = [&](int kb, int sg) {
-#pragma unroll
-        for (int t = 0; t < NI; ++t) {
-            const int u = wave * NI + t, ip = u & 15, pl = u >> 4;
-            const int slot = TERMS == 3 ? pl : 2 * pl;        // Ah, Al, Bh, Bl or Ah, Bh
-            const uint4 *src = (slot & 1) ? Xl : Xh;
-            const int64_t row0 = slot >= 2 ? rowB : rowA;
-            const uint4 *g = src + (size_t)kb * planeStride + (size_t)row0 * 4 + dma_off[t];
-            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(g),
-                                             (__attribute__((address_space(3))) void *)(&lds[sg][slot][64 * ip]),
-                                             16, 0, 0);
-        }
-    };
+    // staging: thread tid moves chunks q_j = tid + 256 j (row q / 4, chunk
+    // q % 4) of every needed plane; global offsets g_j, LDS positions p_j
+    const int g0 = tid, g1 = tid + 256, g2 = tid + 512, g3 = tid + 768;
+#define X3M_POS(q) (((q) >> 2) * 4 + (((q) & 3) ^ (((q) >> 4) & 3)))
+    const int p0 = X3M_POS(g0), p1 = X3M_POS(g1), p2 = X3M_POS(g2), p3 = X3M_POS(g3);
+#undef X3M_POS
+    const uint4 *pAh = Xh + (size_t)rowA * 4, *pAl = Xl + (size_t)rowA * 4;
+    const uint4 *pBh = Xh + (size_t)rowB * 4, *pBl = Xl + (size_t)rowB * 4;
+    uint4 ta0, ta1, ta2, ta3, tb0, tb1, tb2, tb3, tc0, tc1, tc2, tc3, td0, td1, td2, td3;
+#define X3M_LD4(P, ptr, base)           \
+    t##P##0 = (ptr)[(base) + g0];       \
+    t##P##1 = (ptr)[(base) + g1];       \
+    t##P##2 = (ptr)[(base) + g2];       \
+    t##P##3 = (ptr)[(base) + g3]
+#define X3M_ST4(P, sg, slot)            \
+    lds[sg][slot][p0] = t##P##0;        \
+    lds[sg][slot][p1] = t##P##1;        \
+    lds[sg][slot][p2] = t##P##2;        \
+    lds[sg][slot][p3] = t##P##3
+#define X3M_LD(kbv)                                            \
+    do {                                                       \
+        const size_t base_ = (size_t)(kbv) * planeStride;      \
+        X3M_LD4(a, pAh, base_);                                \
+        X3M_LD4(c, pBh, base_);                                \
+        if (TERMS == 3) {                                      \
+            X3M_LD4(b, pAl, base_);                            \
+            X3M_LD4(d, pBl, base_);                            \
+        }                                                      \
+    } while (0)
+#define X3M_ST(sg)                                             \
+    do {                                                       \
+        X3M_ST4(a, sg, 0);                                     \
+        X3M_ST4(c, sg, 2);                                     \
+        if (TERMS == 3) {                                      \
+            X3M_ST4(b, sg, 1);                                 \
+            X3M_ST4(d, sg, 3);                                 \
+        }                                                      \
+    } while (0)
 
     f16v acc[4][4];
 #pragma unroll
@@ -226,69 +240,69 @@ __global__ void __launch_bounds__(256, 1)
 #pragma unroll
             for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
 
-    if (kb0 < kb1) dma(kb0, 0);
-    for (int i = tid; i < kb1 - kb0; i += 256) sgl[i] = sgn[kb0 + i];
-    __builtin_amdgcn_s_waitcnt(0);
+    if (kb0 < kb1) {
+        X3M_LD(kb0);
+        X3M_ST(0);
+        for (int i = tid; i < kb1 - kb0; i += 256) sgl[i] = sgn[kb0 + i];
+    }
     __syncthreads();
     int s = 0;
     for (int kb = kb0; kb < kb1; ++kb) {
+        // next step's chunks (the last step re-reads its own, unused)
+        X3M_LD(kb + 1 < kb1 ? kb + 1 : kb);
+        __builtin_amdgcn_sched_barrier(0);
         const unsigned sw = sgl[kb - kb0];
-        // both substeps' operands first: the compiler makes LDS reads wait
-        // for outstanding LDS DMA, so the DMA of step kb+1 is issued after
-        // the last read of step kb and has the step's MFMAs to complete
-        bf16x8 ch[2][4], cl[2][4], rh[2][4], rl[2][4];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            // column side (MFMA A operand, signed) and row side (B operand)
             const int c = 2 * u + kh;
+            // column side (MFMA A operand) takes the weight signs
             const unsigned bits = (sw >> (16 * u + 8 * kh)) & 0xffu;
             uint4 fl;
             fl.x = ((bits & 1u) << 15) | (((bits >> 1) & 1u) << 31);
             fl.y = (((bits >> 2) & 1u) << 15) | (((bits >> 3) & 1u) << 31);
             fl.z = (((bits >> 4) & 1u) << 15) | (((bits >> 5) & 1u) << 31);
             fl.w = (((bits >> 6) & 1u) << 15) | (((bits >> 7) & 1u) << 31);
+            bf16x8 ch[4], cl[4], rh[4], rl[4];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int r = wc * 128 + 32 * b + l32;
                 const int p = r * 4 + (c ^ ((r >> 2) & 3));
                 uint4 v = lds[s][2][p];
                 v.x ^= fl.x; v.y ^= fl.y; v.z ^= fl.z; v.w ^= fl.w;
-                ch[u][b] = __builtin_bit_cast(bf16x8, v);
+                ch[b] = __builtin_bit_cast(bf16x8, v);
                 if (TERMS == 3) {
                     uint4 vl = lds[s][3][p];
                     vl.x ^= fl.x; vl.y ^= fl.y; vl.z ^= fl.z; vl.w ^= fl.w;
-                    cl[u][b] = __builtin_bit_cast(bf16x8, vl);
+                    cl[b] = __builtin_bit_cast(bf16x8, vl);
                 }
             }
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
                 const int r = wr * 128 + 32 * a + l32;
                 const int p = r * 4 + (c ^ ((r >> 2) & 3));
-                rh[u][a] = __builtin_bit_cast(bf16x8, lds[s][0][p]);
-                if (TERMS == 3) rl[u][a] = __builtin_bit_cast(bf16x8, lds[s][1][p]);
+                rh[a] = __builtin_bit_cast(bf16x8, lds[s][0][p]);
+                if (TERMS == 3) rl[a] = __builtin_bit_cast(bf16x8, lds[s][1][p]);
             }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        if (kb + 1 < kb1) dma(kb + 1, s ^ 1);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
 #pragma unroll
             for (int term = 0; term < TERMS; ++term)
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
                     for (int b = 0; b < 4; ++b) {
-                        const bf16x8 x = term == 2 ? cl[u][b] : ch[u][b];
-                        const bf16x8 y = term == 1 ? rl[u][a] : rh[u][a];
+                        const bf16x8 x = term == 2 ? cl[b] : ch[b];
+                        const bf16x8 y = term == 1 ? rl[a] : rh[a];
                         acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc[a][b], 0, 0, 0);
                     }
-        // the wait for the DMA and the barrier stay behind the MFMAs
+        }
         __builtin_amdgcn_sched_barrier(0);
-        __builtin_amdgcn_s_waitcnt(0);
+        X3M_ST(s ^ 1);
         __syncthreads();
         s ^= 1;
     }
+#undef X3M_LD4
+#undef X3M_ST4
+#undef X3M_LD
+#undef X3M_ST
 
     // D(j, i) of block (a, b): element e of lane l has j = 8 (e/4) + 4 kh + e%4
     // (column within the block), i = l32 (row within the block).  One block
