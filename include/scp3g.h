@@ -113,6 +113,20 @@ int scp3g_accumulate(scp3g_ctx *ctx, int64_t skip, int64_t npts);
  */
 int scp3g_accumulate_normals(scp3g_ctx *ctx, const double *normals_host, int64_t npts);
 
+/*
+ * Potentials on the CPU (WHBB, HBB2-pol or any h2o::PES, PES.h:13-23): the
+ * Cartesians the reference hands to the potential for Sobol seeds
+ * skip .. skip+npts-1 (SCP3.cpp:196-208: y = BSM(Sobol) / sqrt(2),
+ * r = bohr (x0 + MUa y)), crd_host[npts x 9nw] point-major, O H H, A ...
+ */
+int scp3g_points_coords(scp3g_ctx *ctx, int64_t skip, int64_t npts, double *crd_host);
+/*
+ * ... and their energies E_host[npts] (kcal/mol, from the caller's
+ * potential) accumulated into the running sum: bV, basis rows and the SYRK
+ * on the GPU (SCP3.cpp:213-231 with the PES call made by the caller).
+ */
+int scp3g_accumulate_energies(scp3g_ctx *ctx, int64_t skip, int64_t npts, const double *E_host);
+
 /* Number of points accumulated since the last reset. */
 int scp3g_points_done(const scp3g_ctx *ctx, int64_t *npts);
 

@@ -45,6 +45,8 @@ _SIGS = {
     "scp3g_set_mixed_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_cv_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_accumulate_surrogate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
+    "scp3g_points_coords": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
+    "scp3g_accumulate_energies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
     "scp3g_reset": (ctypes.c_int, [ctypes.c_void_p]),
     "scp3g_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "scp3g_accumulate_normals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
@@ -277,6 +279,17 @@ class Context:
 
     def accumulate_cv(self, skip_nn, P_nn, skip_t, P_t):
         self.check(self.L.scp3g_accumulate_cv(self._h, int(skip_nn), int(P_nn), int(skip_t), int(P_t)))
+
+    def points_coords(self, skip, npts):
+        """Cartesians (A, npts x 9nw) of Sobol seeds skip.. for a CPU potential"""
+        crd = np.zeros((npts, 9 * self.nw), dtype=np.float64)
+        self.check(self.L.scp3g_points_coords(self._h, int(skip), int(npts), _ptr(crd)))
+        return crd
+
+    def accumulate_energies(self, skip, E):
+        """accumulate Sobol seeds skip.. with caller energies E (kcal/mol)"""
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        self.check(self.L.scp3g_accumulate_energies(self._h, int(skip), E.size, _ptr(E)))
 
     def accumulate_surrogate(self, skip, npts):
         self.check(self.L.scp3g_accumulate_surrogate(self._h, int(skip), int(npts)))

@@ -463,8 +463,9 @@ int scp3g_reset(scp3g_ctx *ctx)
 
 // one tile: y already in ctx->d_y (nmodes0 x tile, halved), np points.
 // wmode 0: plain weights bV (SCP3.cpp:219-223); 1: surrogate weights w_nn;
-// 2: control-variate correction bV - w_nn.  Modes 1 and 2 multiply the
-// weights by `scale`; the normaliser grows by np*scale for modes 0 and 1.
+// 2: control-variate correction bV - w_nn; 3: plain weights from energies
+// the caller put in ctx->d_E.  Modes 1 and 2 multiply the weights by
+// `scale`; the normaliser grows by np for modes 0 and 3, np*scale for 1.
 }  // extern "C"
 
 int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
@@ -480,6 +481,7 @@ int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
         if (rc) return rc;
         goto basis;
     }
+    if (wmode == 3) goto weights;  // the caller's energies are in d_E
     timer_begin(ctx, "map", &ev);
     rc = check_hip(ctx, launch_map(ctx, ctx->d_y, T, np, ctx->d_crd, T, fp32), "map");
     timer_end(ctx, "map", ev);
@@ -491,6 +493,7 @@ int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
         rc = check_hip(ctx, launch_qtip4pf(ctx, ctx->nw, np, ctx->d_crd, T, fp32, ctx->d_E), "pes");
     timer_end(ctx, "pes", ev);
     if (rc) return rc;
+weights:
     if (wmode == 2) {
         timer_begin(ctx, "mlp", &ev);
         rc = check_hip(ctx, launch_mlp_weights(ctx, ctx->d_y, T, np, ctx->d_E, 1, scale, ctx->d_w), "mlp");
@@ -532,7 +535,7 @@ basis:
         if (rc) return rc;
     }
     ctx->points_done += np;
-    if (wmode == 0) ctx->norm_sum += (double)np;
+    if (wmode == 0 || wmode == 3) ctx->norm_sum += (double)np;
     else if (wmode == 1) ctx->norm_sum += (double)np * scale;
     return 0;
 }
@@ -592,6 +595,59 @@ int scp3g_accumulate_normals(scp3g_ctx *ctx, const double *normals, int64_t npts
                             "upload normals")))
             return rc;
         if ((rc = run_tile(ctx, np, 0, 1.0))) return rc;
+    }
+    return 0;
+}
+
+int scp3g_points_coords(scp3g_ctx *ctx, int64_t skip, int64_t npts, double *crd_host)
+{
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (npts < 0 || skip < 0 || (!crd_host && npts > 0)) return fail(ctx, -4, "bad arguments");
+    if (skip + npts + 64 * 16 >= ((int64_t)1 << 60)) return fail(ctx, -4, "SEED too large");
+    hipSetDevice(ctx->device);
+    if ((rc = ensure_work(ctx))) return rc;
+    const int64_t T = ctx->work_tile;
+    const int n3 = 9 * ctx->nw;
+    std::vector<double> ct((size_t)n3 * T);
+    for (int64_t done = 0; done < npts; done += T) {
+        const int64_t np = std::min<int64_t>(T, npts - done);
+        rc = check_hip(ctx, launch_sobol(ctx, ctx->d_sobol, ctx->nmodes0, skip + done, np, 1, ctx->d_y, T, nullptr),
+                       "sobol");
+        if (!rc) rc = check_hip(ctx, launch_map(ctx, ctx->d_y, T, np, ctx->d_crd, T, 0), "map");
+        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (!rc)
+            rc = check_hip(ctx, hipMemcpy(ct.data(), ctx->d_crd, ct.size() * 8, hipMemcpyDeviceToHost), "coords");
+        if (rc) return rc;
+        // [coordinate][point] -> point-major, 9nw per point (OHH, A)
+        for (int64_t i = 0; i < np; ++i)
+            for (int c = 0; c < n3; ++c) crd_host[(size_t)(done + i) * n3 + c] = ct[(size_t)c * T + i];
+    }
+    return 0;
+}
+
+int scp3g_accumulate_energies(scp3g_ctx *ctx, int64_t skip, int64_t npts, const double *E_host)
+{
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (npts < 0 || skip < 0 || (!E_host && npts > 0)) return fail(ctx, -4, "bad arguments");
+    if (skip + npts + 64 * 16 >= ((int64_t)1 << 60)) return fail(ctx, -4, "SEED too large");
+    hipSetDevice(ctx->device);
+    if ((rc = ensure_sums(ctx))) return rc;
+    if ((rc = ensure_work(ctx))) return rc;
+    const int64_t T = ctx->work_tile;
+    for (int64_t done = 0; done < npts; done += T) {
+        const int64_t np = std::min<int64_t>(T, npts - done);
+        rc = check_hip(ctx, launch_sobol(ctx, ctx->d_sobol, ctx->nmodes0, skip + done, np, 1, ctx->d_y, T, nullptr),
+                       "sobol");
+        if (!rc)
+            rc = check_hip(ctx, hipMemcpyAsync(ctx->d_E, E_host + done, (size_t)np * 8, hipMemcpyHostToDevice,
+                                               ctx->stream),
+                           "upload energies");
+        if (!rc) rc = run_tile(ctx, np, 3, 1.0);
+        // the host buffer is read asynchronously: finish before the next tile
+        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (rc) return rc;
     }
     return 0;
 }

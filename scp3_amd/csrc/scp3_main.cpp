@@ -13,7 +13,10 @@
 //                       estimator has no GPU path)
 //   -B/--block-width w  points per block between checkpoints (default 1024)
 // Additions: --precision fp64|mixed (default fp64), --levels k (lowest k
-// levels printed at the end, default 10, 0 = none), --device d.
+// levels printed at the end, default 10, 0 = none), --device d, --cpu-pes
+// (the potential is evaluated on the host through h2o::PES, one instance
+// per thread, as the reference does; sampling, basis and SYRK stay on the
+// GPU: the path for potentials without a device kernel, such as WHBB).
 // Files: omega0.dat, E0_sd.dat (every 2^14 points), E0_t.dat (empty),
 // sM_%07d.h5 (every 2^17 points), levels.dat (this driver's final levels).
 #include <getopt.h>
@@ -25,6 +28,7 @@
 #include <iomanip>
 #include <vector>
 
+#include "../../include/PES.h"
 #include "../../include/scp3g.h"
 #include "cli_common.h"
 
@@ -35,7 +39,7 @@ struct Opts {
     int64_t skip = (int64_t)1 << 30;
     std::string rng_file, pes = "qtip4pf", input, modes_desc;
     int n2 = 0, n3 = 0, bw = 1024, levels = 10, device = 0;
-    bool gradient = true, have_modes = false;
+    bool gradient = true, have_modes = false, cpu_pes = false;
     int precision = SCP3G_FP64;
 };
 
@@ -78,6 +82,7 @@ Opts parse(int argc, char **argv)
                           {"precision", required_argument, nullptr, 1000},
                           {"levels", required_argument, nullptr, 1001},
                           {"device", required_argument, nullptr, 1002},
+                          {"cpu-pes", no_argument, nullptr, 1003},
                           {nullptr, 0, nullptr, 0}};
     Opts o;
     int ch;
@@ -112,6 +117,7 @@ Opts parse(int argc, char **argv)
             break;
         case 1001: o.levels = atoi(optarg); break;
         case 1002: o.device = atoi(optarg); break;
+        case 1003: o.cpu_pes = true; break;
         default: die("Unknown option: " + std::to_string(ch));
         }
     }
@@ -132,7 +138,15 @@ int main(int argc, char **argv)
 {
     Opts o = parse(argc, argv);
     if (o.pes == "whbb" || o.pes == "hbb2-pol") die("Support for Bowman's WHBB was not included.");
-    if (o.pes != "qtip4pf" && o.pes != "ttm3f") die("Unknown H2O potential: " + o.pes);
+    if (o.cpu_pes) {
+        // h2o::PESFromString decides which host potentials exist
+        h2o::PES *probe = h2o::PESFromString(o.pes);
+        if (!probe) die("Unknown H2O potential: " + o.pes);
+        delete probe;
+        if (!o.rng_file.empty()) die("--cpu-pes takes Sobol points only (no -r)");
+    } else if (o.pes != "qtip4pf" && o.pes != "ttm3f") {
+        die("Unknown H2O potential: " + o.pes);
+    }
     if (o.gradient)
         die("The gradient estimator (the default without -G) has no GPU path in this build; "
             "run with -G for the B^T diag(V) B estimator.");
@@ -163,7 +177,7 @@ int main(int argc, char **argv)
     if (scp3g_create(&ctx, o.device)) die("no usable GPU");
     check(ctx, scp3g_set_system(ctx, N / 3, in.x0.data(), nm0, omega.data(), MUa.data()), "set_system");
     check(ctx, scp3g_set_basis(ctx, (int)modes.size(), modes.data(), o.n2, o.n3), "set_basis");
-    check(ctx, scp3g_set_pes(ctx, o.pes.c_str()), "set_pes");
+    if (!o.cpu_pes) check(ctx, scp3g_set_pes(ctx, o.pes.c_str()), "set_pes");
     check(ctx, scp3g_set_precision(ctx, o.precision), "set_precision");
     check(ctx, scp3g_reset(ctx), "reset");
     int64_t n = 0, n1 = 0, n2s = 0;
@@ -191,7 +205,20 @@ int main(int argc, char **argv)
         do {
             span += o.bw;
         } while (i + span < o.nsobol && (i + span) % log_every != 0);
-        if (rng.is_open()) {
+        if (o.cpu_pes) {
+            // SCP3.cpp:205-225 with the potential on the host
+            const int n3 = (int)(3 * N);
+            std::vector<double> crd((size_t)span * n3), E((size_t)span);
+            check(ctx, scp3g_points_coords(ctx, o.skip + i, span, crd.data()), "coords");
+#pragma omp parallel
+            {
+                h2o::PES *pes = h2o::PESFromString(o.pes);
+#pragma omp for schedule(static)
+                for (int64_t j = 0; j < span; ++j) E[j] = (*pes)((size_t)(N / 3), crd.data() + (size_t)j * n3);
+                delete pes;
+            }
+            check(ctx, scp3g_accumulate_energies(ctx, o.skip + i, span, E.data()), "accumulate");
+        } else if (rng.is_open()) {
             std::vector<double> nr((size_t)span * nm0);
             for (double &v : nr) rng >> v;
             if (!rng) die("rng-file ended early");

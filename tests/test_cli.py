@@ -71,3 +71,24 @@ def test_scp3_run_and_eigensolver(tmp_path):
     np.testing.assert_allclose(evs, ev, rtol=0, atol=2e-7)  # FP32 storage of H
     vec = np.array([float(v) for v in txt[7].split()])
     assert vec.size == 91 and abs(np.linalg.norm(vec) - 1) < 1e-6
+
+def test_cpu_pes_rejects_rng_file_and_unknown_names(tmp_path):
+    r = run(["SCP3", "-G", "--cpu-pes", "-p", "nonsense", DIMER], str(tmp_path))
+    assert r.returncode != 0 and "Unknown H2O potential" in r.stderr
+    r = run(["SCP3", "-G", "--cpu-pes", "-r", "normals.txt", DIMER], str(tmp_path))
+    assert r.returncode != 0 and "--cpu-pes" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cpu_pes_run_matches_device_run(tmp_path):
+    """--cpu-pes (h2o::PES on the host, the path of WHBB) gives the levels of
+    the device-potential run on the same Sobol points."""
+    print("cli case: dimer --cpu-pes -N 15", flush=True)
+    out = {}
+    for tag, extra in (("dev", []), ("cpu", ["--cpu-pes"])):
+        d = tmp_path / tag
+        d.mkdir()
+        r = run(["SCP3", "-G", "-N", "15", "-2", "12", "--levels", "5"] + extra + [DIMER], str(d))
+        assert r.returncode == 0, r.stderr
+        out[tag] = np.loadtxt(d / "levels.dat")
+    np.testing.assert_allclose(out["cpu"][:, 0], out["dev"][:, 0], rtol=0, atol=1e-10)

@@ -283,3 +283,24 @@ def test_mixed_kernels_agree(ctx, hexamer, n2, n3):
         print("n=%d %s: |-slow| %.2e  |-fp64| %.2e (relative to max|H|)" % (out["fp64"].shape[0], k, d_fs, d_f64))
         assert d_fs < 2e-5 and d_f64 < 2e-5
         assert np.all(np.tril(out[k], -1) == 0)
+
+
+def test_cpu_potential_path_matches_device_path(ctx, dimer):
+    """A potential evaluated by the caller (here the oracle's q-TIP4P/F on
+    the CPU, standing in for WHBB behind h2o::PES) through
+    scp3g_points_coords + scp3g_accumulate_energies gives the H of the
+    device-potential path on the same Sobol points."""
+    setup_system(ctx, dimer, tile=1024)
+    skip, P = 1 << 30, 2500
+    crd = ctx.points_coords(skip, P)
+    r_ref = scp3_amd.BOHR * (dimer["x0"][None, :] + oracle.sobol_normals(12, skip, P) @ dimer["MUa"].T)
+    np.testing.assert_allclose(crd, r_ref, rtol=1e-13, atol=1e-13)
+    E = np.array([oracle.qtip4pf(crd[i]) for i in range(P)])
+    ctx.reset()
+    ctx.accumulate_energies(skip, E)
+    H_cpu = ctx.h_download(raw=True)
+    ctx.reset()
+    ctx.accumulate(skip, P)
+    H_dev = ctx.h_download(raw=True)
+    mx, fro = _rel(H_cpu, H_dev)
+    assert mx < 1e-11 and fro < 1e-11, (mx, fro)
