@@ -287,6 +287,7 @@ void scp3g_destroy(scp3g_ctx *ctx)
     mlp_release(ctx);
     release_fast_tiles(ctx);
     release_tiles128(ctx);
+    release_davidson(ctx);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -32,8 +32,8 @@ int64_t lda_of(int64_t n) { return (n + kLdA - 1) / kLdA * kLdA; }
 
 // A(r, c) = Mout(min, max) / P (+ diag on r == c) from the upper-triangle sum
 // H, through 32 x 32 LDS tiles so that both the read of the mirrored
-// triangle and the write stay coalesced.  A has leading dimension lda; its
-// padding rows n..lda-1 are left as they are (zero from allocation).
+// triangle and the write stay coalesced.  A has leading dimension lda
+// (a multiple of 32); its padding rows n..lda-1 are set to zero.
 __global__ void __launch_bounds__(256)
     k_sym_full(const double *__restrict__ H, int64_t ldh, int64_t n, double P,
                const double *__restrict__ diag, double *__restrict__ A, int64_t lda)
@@ -50,6 +50,8 @@ __global__ void __launch_bounds__(256)
                 if (r > c) v = H[(size_t)r * ldh + c] / P;
                 if (r == c) v += diag[r];
                 A[(size_t)c * lda + r] = v;
+            } else if (r < lda && c < n) {
+                A[(size_t)c * lda + r] = 0.0;
             }
         }
     } else {
@@ -62,7 +64,7 @@ __global__ void __launch_bounds__(256)
         __syncthreads();
         for (int k = ty; k < 32; k += 8) {
             const int64_t r = br + tx, c = bc + k;
-            if (r < n && c < n) A[(size_t)c * lda + r] = t[tx][k];
+            if (r < lda && c < n) A[(size_t)c * lda + r] = r < n ? t[tx][k] : 0.0;
         }
     }
 }
@@ -165,20 +167,30 @@ __global__ void __launch_bounds__(256)
     Y[(size_t)j * ldy + r] = s;
 }
 
-// C(p x q) = P^T Q for tall P (n x p, ld ldp) and Q (n x q, ld ldq).  One
-// wave owns a 4 x 4 block of C: lanes stride over n, then a butterfly of
-// shuffles reduces the 16 sums (fixed order, no LDS, no barrier; the
-// Davidson sizes make this a latency problem, not a bandwidth one).
-__global__ void __launch_bounds__(64)
-    k_gemm_tn(const double *__restrict__ Pm, int64_t ldp, int p, const double *__restrict__ Q,
-              int64_t ldq, int q, int64_t n, double *__restrict__ C, int ldc)
+__global__ void k_diag_of(const double *__restrict__ A, int64_t lda, int64_t n, double *__restrict__ d)
 {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r < n) d[r] = A[(size_t)r * lda + r];
+}
+
+// C(p x q) = P^T Q for tall P = [P0 | P1] (n x (p0 + p1), both with ld ldp)
+// and Q (n x q, ld ldq).  One 256-thread workgroup owns a 4 x 4 block of C:
+// the threads stride over n, each wave reduces its 16 sums with shuffles,
+// and the four wave sums are added in wave order (a fixed order, so results
+// do not vary between runs).
+__global__ void __launch_bounds__(256)
+    k_gemm_tn(const double *__restrict__ P0, int p0, const double *__restrict__ P1, int p1, int64_t ldp,
+              const double *__restrict__ Q, int64_t ldq, int q, int64_t n, double *__restrict__ C, int ldc)
+{
+    __shared__ double red[4][16];
+    const int p = p0 + p1;
     const int i0 = blockIdx.x * 4, j0 = blockIdx.y * 4;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double *pc[4], *qc[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-        pc[a] = Pm + (size_t)(i0 + a < p ? i0 + a : i0) * ldp;
+        const int i = i0 + a < p ? i0 + a : i0;
+        pc[a] = i < p0 ? P0 + (size_t)i * ldp : P1 + (size_t)(i - p0) * ldp;
         qc[a] = Q + (size_t)(j0 + a < q ? j0 + a : j0) * ldq;
     }
     double acc[4][4];
@@ -187,7 +199,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
 #pragma unroll 2
-    for (int64_t k = lane; k < n; k += 64) {
+    for (int64_t k = tid; k < n; k += 256) {
         double pv[4], qv[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -208,113 +220,237 @@ __global__ void __launch_bounds__(64)
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
             acc[a][c] = v;
         }
-    if (lane < 16) {
-        const int a = lane >> 2, c = lane & 3;
-        double v = 0.0;
+    if (lane == 0) {
 #pragma unroll
-        for (int aa = 0; aa < 4; ++aa)
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int cc = 0; cc < 4; ++cc)
-                if (aa == a && cc == c) v = acc[aa][cc];
+            for (int c = 0; c < 4; ++c) red[wave][4 * a + c] = acc[a][c];
+    }
+    __syncthreads();
+    if (tid < 16) {
+        const int a = tid >> 2, c = tid & 3;
+        const double v = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
         if (i0 + a < p && j0 + c < q) C[(size_t)(j0 + c) * ldc + i0 + a] = v;
     }
 }
 
-// Y(n x q) = beta*Y + alpha * P(n x p) S(p x q), q <= kMaxB; S column-major
-// with ld lds.  One wave owns 64 rows and 8 columns: the 8 columns of S are
-// staged in LDS once, each lane streams its row of P (one load per k).
-__global__ void __launch_bounds__(64)
-    k_gemm_nn(const double *__restrict__ Pm, int64_t ldp, int p, const double *__restrict__ S,
-              int lds, int q, double alpha, double beta, double *__restrict__ Y, int64_t ldy,
-              int64_t n)
+// Partial products of one 256-thread workgroup: 64 rows r and 8 columns
+// j0.. of P S for P = [P0 | P1] (p0 + p1 columns, ld ldp) and S (p x q, ld
+// lds, staged in LDS); the four waves take consecutive quarters of the
+// contraction index and their sums meet in LDS, added in wave order.  The
+// result is valid in wave 0 (acc[j] for column j0 + j of row r).
+struct NNBlock {
+    double acc[8];
+};
+__device__ __forceinline__ void nn_block(const double *__restrict__ P0, int p0, const double *__restrict__ P1,
+                                         int p1, int64_t ldp, const double *__restrict__ S, int lds, int q,
+                                         int64_t n, int64_t r, int j0, double *ss, double *red, NNBlock &out)
 {
-    constexpr int kMaxP = 256;
-    __shared__ double ss[kMaxP][8];
-    const int64_t r = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    const int j0 = blockIdx.y * 8;
-    for (int e = threadIdx.x; e < p * 8; e += 64) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int p = p0 + p1;
+    for (int e = tid; e < p * 8; e += 256) {
         const int k = e >> 3, j = e & 7;
-        ss[k][j] = (j0 + j < q) ? S[(size_t)(j0 + j) * lds + k] : 0.0;
+        ss[k * 8 + j] = (j0 + j < q) ? S[(size_t)(j0 + j) * lds + k] : 0.0;
     }
     __syncthreads();
-    if (r >= n) return;
-    double acc[8];
+    const int kq = (p + 3) / 4, k0 = wave * kq, k1 = k0 + kq < p ? k0 + kq : p;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) acc[j] = 0.0;
-#pragma unroll 4
-    for (int k = 0; k < p; ++k) {
-        const double pv = Pm[(size_t)k * ldp + r];
+    for (int j = 0; j < 8; ++j) out.acc[j] = 0.0;
+    if (r < n) {
+#pragma unroll 8
+        for (int k = k0; k < k1; ++k) {
+            const double pv = k < p0 ? P0[(size_t)k * ldp + r] : P1[(size_t)(k - p0) * ldp + r];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc[j] = fma(pv, ss[k][j], acc[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-        if (j0 + j < q) {
-            double *yp = Y + (size_t)(j0 + j) * ldy + r;
-            *yp = (beta == 0.0 ? 0.0 : beta * *yp) + alpha * acc[j];
+            for (int j = 0; j < 8; ++j) out.acc[j] = fma(pv, ss[k * 8 + j], out.acc[j]);
         }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) red[((wave - 1) * 8 + j) * 64 + lane] = out.acc[j];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            out.acc[j] = (out.acc[j] + red[j * 64 + lane]) + (red[(8 + j) * 64 + lane] + red[(16 + j) * 64 + lane]);
+    }
 }
 
-__global__ void k_diag_of(const double *__restrict__ A, int64_t lda, int64_t n, double *__restrict__ d)
-{
-    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (r < n) d[r] = A[(size_t)r * lda + r];
-}
+constexpr int kMaxP = kMaxB + 128;  // contraction length limit of k_gemm_nn / k_ritz
 
-// R = AX - X theta; T = R / (theta - diag) (Davidson correction).
+// Y(n x q) = P S for P = [P0 | P1] (p0 + p1 <= kMaxP columns), q <= kMaxB;
+// S column-major with ld lds.  Y may alias P1 (each workgroup reads all of
+// its rows before it writes them).
 __global__ void __launch_bounds__(256)
-    k_residual(const double *__restrict__ AX, const double *__restrict__ X,
-               const double *__restrict__ theta, const double *__restrict__ diag, int64_t n,
-               int64_t ld, double *__restrict__ R, double *__restrict__ T)
+    k_gemm_nn(const double *P0, int p0, const double *P1, int p1, int64_t ldp, const double *__restrict__ S,
+              int lds, int q, double *Y, int64_t ldy, int64_t n)
 {
-    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int j = blockIdx.y;
-    if (r >= n) return;
-    const size_t idx = (size_t)j * ld + r;
-    const double res = AX[idx] - X[idx] * theta[j];
-    R[idx] = res;
-    double den = theta[j] - diag[r];
-    if (fabs(den) < 1e-8) den = den < 0 ? -1e-8 : 1e-8;
-    T[idx] = res / den;
+    __shared__ double ss[kMaxP * 8];
+    __shared__ double red[3 * 8 * 64];
+    const int64_t r = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j0 = blockIdx.y * 8;
+    NNBlock o;
+    nn_block(P0, p0, P1, p1, ldp, S, lds, q, n, r, j0, ss, red, o);
+    if (threadIdx.x < 64 && r < n) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j0 + j < q) Y[(size_t)(j0 + j) * ldy + r] = o.acc[j];
+    }
 }
 
-// host-side small dense helpers ---------------------------------------------
-
-static bool cholesky_upper(int b, std::vector<double> &G)  // G = U^T U, U upper in place
+// Ritz vectors and Davidson corrections in one pass: X = V S, AX = W S
+// (S: p x b, ld p), R = AX - X theta, T = R / (theta - diag) (theta follows
+// S in memory: theta[j] = S[p * b + j]).  X, R and T are written; the sum
+// of the squares of R over the workgroup's 64 rows of column j goes to
+// part[j * gridDim.x + blockIdx.x].
+__global__ void __launch_bounds__(256)
+    k_ritz(const double *__restrict__ V, const double *__restrict__ W, int p, const double *__restrict__ S,
+           int b, const double *__restrict__ diag, int64_t n, double *__restrict__ X, double *__restrict__ R,
+           double *__restrict__ T, double *__restrict__ part)
 {
-    for (int j = 0; j < b; ++j) {
-        for (int i = 0; i <= j; ++i) {
-            double s = G[(size_t)j * b + i];
-            for (int k = 0; k < i; ++k) s -= G[(size_t)i * b + k] * G[(size_t)j * b + k];
-            if (i == j) {
-                if (s <= 0) return false;
-                G[(size_t)j * b + j] = std::sqrt(s);
-            } else {
-                G[(size_t)j * b + i] = s / G[(size_t)i * b + i];
+    __shared__ double ss[kMaxP * 8];
+    __shared__ double red[3 * 8 * 64];
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 64 + lane;
+    const int j0 = blockIdx.y * 8;
+    NNBlock x, ax;
+    nn_block(V, p, nullptr, 0, n, S, p, b, n, r, j0, ss, red, x);
+    __syncthreads();
+    nn_block(W, p, nullptr, 0, n, S, p, b, n, r, j0, ss, red, ax);
+    if (threadIdx.x >= 64) return;
+    const double dr = r < n ? diag[r] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        double res = 0.0;
+        if (j0 + j < b && r < n) {
+            const double th = S[(size_t)p * b + j0 + j];
+            res = ax.acc[j] - x.acc[j] * th;
+            double den = th - dr;
+            if (fabs(den) < 1e-8) den = den < 0 ? -1e-8 : 1e-8;
+            const size_t idx = (size_t)(j0 + j) * n + r;
+            X[idx] = x.acc[j];
+            R[idx] = res;
+            T[idx] = res / den;
+        }
+        double s2 = res * res;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s2 += __shfl_xor(s2, off);
+        if (lane == 0 && j0 + j < b) part[(size_t)(j0 + j) * gridDim.x + blockIdx.x] = s2;
+    }
+}
+
+// One pass of block Gram-Schmidt plus CholQR for a block T (n x nb) against
+// an orthonormal V (n x p0), from the Gram matrix of [V T] with T:
+// Gf = [V T]^T T ((p0 + nb) x nb, ld p0 + nb), C = V^T T its first p0 rows.
+// Since V^T V = I, (T - V C)^T (T - V C) = T^T T - C^T C = U^T U (Cholesky),
+// and Q = (T - V C) U^-1 = [V T] [-C U^-1; U^-1] is orthonormal; the
+// coefficients Sf = [-C U^-1; U^-1] ((p0 + nb) x nb, ld p0 + nb) are the
+// output.  A column whose Cholesky pivot is at most 1e-12 of its squared
+// norm in T lies in the span of V and its predecessors (to 1e-6 relative):
+// it is dropped, i.e. its coefficient column is zero and Q has a zero
+// column there.  NB >= nb; 256 threads, the factorisation in wave 0 (lane
+// c owns column c; the broadcasts are v_readlane of a fixed lane).
+template <int NB>
+__global__ void __launch_bounds__(256)
+    k_chol_block(const double *__restrict__ Gf, int p0, int nb, double *__restrict__ Sf)
+{
+    __shared__ double Cs[128][NB + 1];
+    __shared__ double G22[NB][NB + 1];
+    __shared__ double U[NB][NB + 1];
+    __shared__ double Ui[NB][NB + 1];
+    const int tid = threadIdx.x;
+    const int ld = p0 + nb;
+    for (int e = tid; e < p0 * nb; e += 256) {
+        const int k = e % p0, c = e / p0;
+        Cs[k][c] = Gf[(size_t)c * ld + k];
+    }
+    __syncthreads();
+    // G22 = T^T T - C^T C
+    for (int e = tid; e < nb * nb; e += 256) {
+        const int rr = e % nb, c = e / nb;
+        double s = 0.0;
+        for (int k = 0; k < p0; ++k) s = fma(Cs[k][rr], Cs[k][c], s);
+        G22[rr][c] = Gf[(size_t)c * ld + p0 + rr] - s;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int c = tid;
+        double col[NB], gref = 0.0;
+#pragma unroll
+        for (int r = 0; r < NB; ++r) col[r] = (c < nb && r < nb) ? G22[r][c] : 0.0;
+        if (c < nb) gref = Gf[(size_t)c * ld + p0 + c];
+        auto bcast = [](double v, int l) {
+            const long long u = __double_as_longlong(v);
+            const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffff), l);
+            const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+            return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+        };
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j < nb) {
+                const double pivot = bcast(col[j], j);
+                const double gjj = bcast(gref, j);
+                const bool keep = pivot > 1e-12 * gjj && pivot > 0.0;
+                const double ujj = keep ? sqrt(pivot) : 0.0;
+                const double ujc = !keep ? 0.0 : (c > j ? col[j] / ujj : (c == j ? ujj : 0.0));
+                if (c < NB) U[j][c] = ujc;
+#pragma unroll
+                for (int r = j + 1; r < NB; ++r) col[r] -= bcast(ujc, r) * ujc;
             }
         }
-        for (int i = j + 1; i < b; ++i) G[(size_t)j * b + i] = 0.0;
     }
-    return true;
-}
-
-static std::vector<double> upper_inverse(int b, const std::vector<double> &U)
-{
-    std::vector<double> Ui((size_t)b * b, 0.0);
-    for (int j = 0; j < b; ++j) {
-        Ui[(size_t)j * b + j] = 1.0 / U[(size_t)j * b + j];
-        for (int i = j - 1; i >= 0; --i) {
-            double s = 0.0;
-            for (int k = i + 1; k <= j; ++k) s += U[(size_t)k * b + i] * Ui[(size_t)j * b + k];
-            Ui[(size_t)j * b + i] = -s / U[(size_t)i * b + i];
+    __syncthreads();
+    // column c of U^-1 by back substitution (zero for dropped rows/columns)
+    if (tid < nb) {
+        const int c = tid;
+        double x[NB];
+#pragma unroll
+        for (int i = NB - 1; i >= 0; --i) {
+            double v = 0.0;
+            if (i <= c) {
+                const double uii = U[i][i];
+                if (uii != 0.0) {
+                    double s = i == c ? 1.0 : 0.0;
+#pragma unroll
+                    for (int k = i + 1; k < NB; ++k)
+                        if (k <= c) s -= U[i][k] * x[k];
+                    v = s / uii;
+                }
+            }
+            x[i] = v;
         }
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+            if (i < nb) Ui[i][c] = x[i];
     }
-    return Ui;
+    __syncthreads();
+    // Sf = [-C U^-1; U^-1]
+    for (int e = tid; e < ld * nb; e += 256) {
+        const int rr = e % ld, c = e / ld;
+        double v;
+        if (rr < p0) {
+            double s = 0.0;
+            for (int k = 0; k < nb; ++k) s = fma(Cs[rr][k], Ui[k][c], s);
+            v = -s;
+        } else {
+            v = Ui[rr - p0][c];
+        }
+        Sf[(size_t)c * ld + rr] = v;
+    }
 }
 
 }  // namespace scp3g
 
 using namespace scp3g;
+
+void scp3g::release_davidson(scp3g_ctx *ctx)
+{
+    if (ctx->d_dav) hipFree(ctx->d_dav);
+    if (ctx->h_dav) hipHostFree(ctx->h_dav);
+    ctx->d_dav = nullptr;
+    ctx->h_dav = nullptr;
+    ctx->dav_bytes = ctx->hdav_bytes = 0;
+}
 
 namespace {
 
@@ -322,7 +458,7 @@ struct Dav {
     scp3g_ctx *ctx;
     int64_t n, lda;
     const double *A;
-    double *V, *W, *X, *AX, *R, *T, *AT, *S, *G, *theta, *diag, *part;
+    double *V, *W, *X, *R, *T, *AT, *S, *SC, *G, *diag, *rr, *part;
     int mmax, b;
 
     hipError_t symm(const double *Xp, int nb, double *Yp)
@@ -350,20 +486,58 @@ struct Dav {
         timer_end(ctx, "symv", ev);
         return hipGetLastError();
     }
-    hipError_t tn(const double *Pm, int p, const double *Q, int q, double *C, int ldc)
+    // C = [P0 | P1]^T Q, ld ldc
+    hipError_t tn(const double *P0, int p0, const double *P1, int p1, const double *Q, int q, double *C, int ldc)
     {
-        hipLaunchKernelGGL(k_gemm_tn, dim3((unsigned)((p + 3) / 4), (unsigned)((q + 3) / 4)), dim3(64), 0,
-                           ctx->stream, Pm, n, p, Q, n, q, n, C, ldc);
+        const int p = p0 + p1;
+        hipLaunchKernelGGL(k_gemm_tn, dim3((unsigned)((p + 3) / 4), (unsigned)((q + 3) / 4)), dim3(256), 0,
+                           ctx->stream, P0, p0, P1, p1, n, Q, n, q, n, C, ldc);
         return hipGetLastError();
     }
-    hipError_t nn(const double *Pm, int p, const double *Sm, int lds, int q, double alpha,
-                  double beta, double *Y)
+    // Y = [P0 | P1] S (S: (p0 + p1) x q, ld lds)
+    hipError_t nn(const double *P0, int p0, const double *P1, int p1, const double *Sm, int lds, int q, double *Y)
     {
-        if (p > 256) return hipErrorInvalidValue;  // k_gemm_nn stages at most 256 rows of S
-        hipLaunchKernelGGL(k_gemm_nn, dim3((unsigned)((n + 63) / 64), (unsigned)((q + 7) / 8)), dim3(64), 0,
-                           ctx->stream, Pm, n, p,
-                           Sm, lds, q, alpha, beta, Y, n, n);
+        if (p0 + p1 > kMaxP || q > kMaxB) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_gemm_nn, dim3((unsigned)((n + 63) / 64), (unsigned)((q + 7) / 8)), dim3(256), 0,
+                           ctx->stream, P0, p0, P1, p1, n, Sm, lds, q, Y, n, n);
         return hipGetLastError();
+    }
+    // X = V S, R, T and the residual partial sums (S: m x b then theta)
+    hipError_t ritz(int m)
+    {
+        if (m > kMaxP) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_ritz, dim3((unsigned)((n + 63) / 64), (unsigned)((b + 7) / 8)), dim3(256), 0,
+                           ctx->stream, V, W, m, S, b, diag, n, X, R, T, rr);
+        return hipGetLastError();
+    }
+    hipError_t chol_block(int p0, int nb)
+    {
+        if (p0 > 128 || nb > kMaxB) return hipErrorInvalidValue;
+        if (nb <= 8)
+            hipLaunchKernelGGL(k_chol_block<8>, dim3(1), dim3(256), 0, ctx->stream, G, p0, nb, S);
+        else if (nb <= 16)
+            hipLaunchKernelGGL(k_chol_block<16>, dim3(1), dim3(256), 0, ctx->stream, G, p0, nb, S);
+        else if (nb <= 24)
+            hipLaunchKernelGGL(k_chol_block<24>, dim3(1), dim3(256), 0, ctx->stream, G, p0, nb, S);
+        else
+            hipLaunchKernelGGL(k_chol_block<kMaxB>, dim3(1), dim3(256), 0, ctx->stream, G, p0, nb, S);
+        return hipGetLastError();
+    }
+    // The block T (n x nb) orthonormalised against V[:, :m] and within
+    // itself into dst: two passes of k_chol_block's Gram-matrix form of
+    // block Gram-Schmidt + CholQR (dropped columns come out zero)
+    hipError_t orth_cholqr(int m, int nb, double *dst)
+    {
+        hipError_t e;
+        const double *src = T;
+        for (int pass = 0; pass < 2; ++pass) {
+            double *out = pass == 0 ? AT : dst;
+            if ((e = tn(V, m, src, nb, src, nb, G, m + nb)) != hipSuccess) return e;
+            if ((e = chol_block(m, nb)) != hipSuccess) return e;
+            if ((e = nn(V, m, src, nb, S, m + nb, nb, out)) != hipSuccess) return e;
+            src = out;
+        }
+        return hipSuccess;
     }
 };
 
@@ -371,6 +545,14 @@ struct Dav {
 
 // Lowest k eigenpairs of A = sym(Hsrc[0:n, 0:n] / P) + diag, where Hsrc is a
 // device upper triangle with leading dimension ldh.
+//
+// One host synchronisation per iteration: the new columns of G = V^T W and
+// the residual norms of the previous iteration arrive together, so the
+// convergence test lags one iteration behind the GPU (the iterate it accepts,
+// X and theta of the previous iteration, is still in place when it is
+// tested).  The correction block is orthonormalised on the GPU (CholQR2 with
+// column dropping); a dropped column stays in V as a zero column, which the
+// host leaves out of the projected problem.
 static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t n, double P,
                         const std::vector<double> &dg, int k, double tol, double *evals,
                         double *evecs)
@@ -381,54 +563,71 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
     hipSetDevice(ctx->device);
     int rc;
     const int b = std::min<int64_t>(k + 4, std::min<int64_t>(kMaxB, n));
-    const int mmax = (int)std::min<int64_t>(n, std::max(4 * b, 64));
+    // subspace limit: a larger subspace takes fewer iterations, the host's
+    // projected eigenproblem costs O(m^3) per iteration (for 10 levels of the
+    // 1225-state hexamer: 27 iterations at 64, 23 at 84, 22 at 128)
+    const int mmax = (int)std::min<int64_t>(n, std::min(128, std::max(4 * b, 64)));
     const size_t nn_ = (size_t)n;
     const int64_t lda = lda_of(n);
-    double *d_A = nullptr, *d_buf = nullptr;
-    const size_t nbuf = nn_ * (2 * mmax + 5 * kMaxB) + (size_t)mmax * mmax * 2 + 2 * nn_ +
+    // device work space (kept in the context between solves)
+    const size_t nA = (size_t)lda * nn_;
+    // residual partial sums: 64-row chunks per column
+    const int nchunk = (int)((n + 63) / 64);
+    const size_t nrr = (size_t)kMaxB * nchunk, nG = (size_t)mmax * mmax, nS = (size_t)mmax * kMaxB + kMaxB,
+                 nC = (size_t)mmax * 2 * kMaxB;
+    const size_t nbuf = nn_ * (2 * mmax + 4 * kMaxB) + std::max(nS, nG) + nrr + nG + nC + nn_ +
                         (size_t)kMaxSplit * kMaxB * nn_;
-    if ((rc = check_hip(ctx, hipMalloc(&d_A, (size_t)lda * nn_ * 8), "alloc A"))) return rc;
-    if ((rc = check_hip(ctx, hipMalloc(&d_buf, nbuf * 8), "alloc davidson"))) {
-        hipFree(d_A);
-        return rc;
+    const size_t dbytes = (nA + nbuf) * 8;
+    if (ctx->dav_bytes < dbytes) {
+        if (ctx->d_dav) hipFree(ctx->d_dav);
+        ctx->d_dav = nullptr;
+        ctx->dav_bytes = 0;
+        if ((rc = check_hip(ctx, hipMalloc(&ctx->d_dav, dbytes), "alloc davidson"))) return rc;
+        ctx->dav_bytes = dbytes;
     }
+    // pinned host staging: [residual partial sums | G columns] (one
+    // download), [S | theta] (one upload), C
+    const size_t hbytes = (nrr + nG + nS + nC) * 8;
+    if (ctx->hdav_bytes < hbytes) {
+        if (ctx->h_dav) hipHostFree(ctx->h_dav);
+        ctx->h_dav = nullptr;
+        ctx->hdav_bytes = 0;
+        if ((rc = check_hip(ctx, hipHostMalloc(reinterpret_cast<void **>(&ctx->h_dav), hbytes), "pinned davidson")))
+            return rc;
+        ctx->hdav_bytes = hbytes;
+    }
+    double *const pRR = ctx->h_dav, *const pG = pRR + nrr, *const pS = pG + nG, *const pC = pS + nS;
     Dav dv;
     dv.ctx = ctx;
     dv.n = n;
     dv.lda = lda;
-    dv.A = d_A;
     dv.mmax = mmax;
     dv.b = b;
-    double *p = d_buf;
+    double *p = ctx->d_dav;
+    double *d_A = p;
+    dv.A = d_A; p += nA;
     dv.V = p; p += nn_ * mmax;
     dv.W = p; p += nn_ * mmax;
     dv.X = p; p += nn_ * kMaxB;
-    dv.AX = p; p += nn_ * kMaxB;
     dv.R = p; p += nn_ * kMaxB;
     dv.T = p; p += nn_ * kMaxB;
     dv.AT = p; p += nn_ * kMaxB;
-    dv.S = p; p += (size_t)mmax * mmax;
-    dv.G = p; p += (size_t)mmax * mmax;
-    dv.theta = p; p += nn_;
+    dv.S = p; p += std::max(nS, nG);
+    dv.rr = p; p += nrr;
+    dv.G = p; p += nG;  // directly behind rr: one download takes both
+    dv.SC = p; p += nC;
     dv.diag = p; p += nn_;
     dv.part = p; p += (size_t)kMaxSplit * kMaxB * nn_;
 
-    auto cleanup = [&]() {
-        hipStreamSynchronize(ctx->stream);
-        hipFree(d_A);
-        hipFree(d_buf);
-    };
 #define DV_CHECK(expr, what)                                    \
     do {                                                        \
         if ((rc = check_hip(ctx, (expr), what))) {              \
-            cleanup();                                          \
+            hipStreamSynchronize(ctx->stream);                  \
             return rc;                                          \
         }                                                       \
     } while (0)
 
     DV_CHECK(hipMemcpyAsync(dv.diag, dg.data(), nn_ * 8, hipMemcpyHostToDevice, ctx->stream), "diag");
-    // A's padding rows n..lda-1 take part in the k-chunks of A X: zero them
-    DV_CHECK(hipMemsetAsync(d_A, 0, (size_t)lda * nn_ * 8, ctx->stream), "clear A");
     {
         hipEvent_t ev;
         timer_begin(ctx, "symfill", &ev);
@@ -445,7 +644,7 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
     if (n < 4 * b) {
         // too small for a subspace iteration (the restarted subspace plus a
         // block must fit in n): solve the dense problem on the host
-        std::vector<double> Ah((size_t)lda * nn_), Gd(nn_ * nn_), w, Z;
+        std::vector<double> Ah(nA), Gd(nn_ * nn_), w, Z;
         DV_CHECK(hipMemcpyAsync(Ah.data(), d_A, Ah.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "A");
         DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
         for (int64_t c = 0; c < n; ++c)
@@ -453,7 +652,6 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
         small_eigh((int)n, Gd, k, w, Z);
         for (int j = 0; j < k; ++j) evals[j] = w[j];
         if (evecs) std::copy(Z.begin(), Z.begin() + nn_ * k, evecs);
-        cleanup();
         return 0;
     }
     std::vector<double> dA(nn_);
@@ -472,25 +670,38 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
         DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
     }
     // G = V^T W is kept on the host (ld mmax) and extended by the columns
-    // of each new block; after a restart it is recomputed (keep x keep).
+    // of each new block; after a restart it is recomputed.  live[c] marks
+    // the columns of V that are not zero (dropped by CholQR).
     int m = b, m0 = 0;
     DV_CHECK(dv.symm(dv.V, b, dv.W), "AV");
-    std::vector<double> Gh((size_t)mmax * mmax, 0.0), Cn, theta, Sb, Sp, Gm;
+    DV_CHECK(dv.tn(dv.V, m, nullptr, 0, dv.W, m, dv.G, m), "VtW");
+    DV_CHECK(hipMemcpyAsync(pG, dv.G, (size_t)m * m * 8, hipMemcpyDeviceToHost, ctx->stream), "G");
+    std::vector<double> Gh((size_t)mmax * mmax, 0.0), theta, Sb, Sp, Gm, thetaPrev;
+    std::vector<char> live((size_t)mmax, 1);
+    std::vector<int> idx;
     int sp_rows = 0;  // rows of Sp (previous Ritz coefficients), 0 = none
-    std::vector<double> resn(b);
-    int iter = 0;
-    bool converged = false;
+    bool have_prev = false, converged = false, fallback_done = false;
     const int maxiter = 400;
-    for (; iter < maxiter; ++iter) {
+    for (int iter = 0; iter < maxiter; ++iter) {
+        DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
+        // residual norms of the previous iteration's Ritz pairs
+        if (have_prev) {
+            double worst = 0.0;
+            for (int j = 0; j < k; ++j) {
+                double s2 = 0.0;
+                for (int q = 0; q < nchunk; ++q) s2 += pRR[(size_t)j * nchunk + q];
+                worst = std::max(worst, std::sqrt(s2));
+            }
+            if (worst < tol) {
+                converged = true;
+                break;
+            }
+        }
         // new columns m0..m-1 of G
         const int nc = m - m0;
-        DV_CHECK(dv.tn(dv.V, m, dv.W + (size_t)m0 * n, nc, dv.G, m), "VtW");
-        Cn.resize((size_t)m * nc);
-        DV_CHECK(hipMemcpyAsync(Cn.data(), dv.G, Cn.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "G");
-        DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
         for (int c = 0; c < nc; ++c)
             for (int r = 0; r < m; ++r) {
-                const double v = Cn[(size_t)c * m + r];
+                const double v = pG[(size_t)c * m + r];
                 Gh[(size_t)(m0 + c) * mmax + r] = v;
                 if (r < m0) Gh[(size_t)r * mmax + m0 + c] = v;
             }
@@ -499,30 +710,62 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
                 const double v = 0.5 * (Gh[(size_t)c * mmax + r] + Gh[(size_t)r * mmax + c]);
                 Gh[(size_t)c * mmax + r] = Gh[(size_t)r * mmax + c] = v;
             }
-        Gm.resize((size_t)m * m);
-        for (int c = 0; c < m; ++c)
-            std::copy(Gh.begin() + (size_t)c * mmax, Gh.begin() + (size_t)c * mmax + m, Gm.begin() + (size_t)c * m);
-        small_eigh(m, Gm, b, theta, Sb);
-        // X = V S_b, AX = W S_b
-        DV_CHECK(hipMemcpyAsync(dv.S, Sb.data(), (size_t)m * b * 8, hipMemcpyHostToDevice, ctx->stream), "S");
-        DV_CHECK(hipMemcpyAsync(dv.theta, theta.data(), (size_t)b * 8, hipMemcpyHostToDevice, ctx->stream),
-                 "theta");
-        DV_CHECK(dv.nn(dv.V, m, dv.S, m, b, 1.0, 0.0, dv.X), "X");
-        DV_CHECK(dv.nn(dv.W, m, dv.S, m, b, 1.0, 0.0, dv.AX), "AX");
-        hipLaunchKernelGGL(k_residual, dim3((unsigned)((n + 255) / 256), (unsigned)b), dim3(256), 0,
-                           ctx->stream, dv.AX, dv.X, dv.theta, dv.diag, n, n, dv.R, dv.T);
-        DV_CHECK(hipGetLastError(), "residual");
-        // residual norms
-        DV_CHECK(dv.tn(dv.R, b, dv.R, b, dv.G, b), "RtR");
-        std::vector<double> RR((size_t)b * b);
-        DV_CHECK(hipMemcpyAsync(RR.data(), dv.G, RR.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "RR");
-        DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
-        double worst = 0.0;
-        for (int j = 0; j < b; ++j) {
-            resn[j] = std::sqrt(std::max(0.0, RR[(size_t)j * b + j]));
-            if (j < k) worst = std::max(worst, resn[j]);
+        // a dropped (zero) column of V has a zero column in G
+        int nlive_new = 0;
+        for (int c = m0; c < m; ++c) {
+            live[c] = Gh[(size_t)c * mmax + c] != 0.0;
+            nlive_new += live[c];
         }
-        if (worst < tol || m >= n) {
+        if (nlive_new == 0 && m0 > 0) {
+            // the whole correction block was dropped: fall back once to the
+            // previous residuals, orthonormalised against V
+            if (fallback_done) {
+                rc = fail(ctx, -8, "Davidson: search space collapsed");
+                hipStreamSynchronize(ctx->stream);
+                return rc;
+            }
+            fallback_done = true;
+            m = m0;
+            DV_CHECK(hipMemcpyAsync(dv.T, dv.R, nn_ * b * 8, hipMemcpyDeviceToDevice, ctx->stream), "T=R");
+            DV_CHECK(dv.orth_cholqr(m, b, dv.V + (size_t)m * n), "R orth");
+            DV_CHECK(dv.symm(dv.V + (size_t)m * n, b, dv.W + (size_t)m * n), "AR");
+            DV_CHECK(dv.tn(dv.V, m + b, nullptr, 0, dv.W + (size_t)m * n, b, dv.G, m + b), "VtW");
+            DV_CHECK(hipMemcpyAsync(pG, dv.G, (size_t)(m + b) * b * 8, hipMemcpyDeviceToHost, ctx->stream), "G");
+            m0 = m;
+            m += b;
+            --iter;
+            continue;
+        }
+        fallback_done = false;
+        idx.clear();
+        for (int c = 0; c < m; ++c)
+            if (live[c]) idx.push_back(c);
+        const int ml = (int)idx.size();
+        if (ml < b) {
+            rc = fail(ctx, -8, "Davidson: search space collapsed");
+            hipStreamSynchronize(ctx->stream);
+            return rc;
+        }
+        Gm.resize((size_t)ml * ml);
+        for (int c = 0; c < ml; ++c)
+            for (int r = 0; r < ml; ++r) Gm[(size_t)c * ml + r] = Gh[(size_t)idx[c] * mmax + idx[r]];
+        std::vector<double> Sl;
+        small_eigh(ml, Gm, b, theta, Sl);
+        // coefficients on all m columns (zero rows for the dropped ones)
+        Sb.assign((size_t)m * b, 0.0);
+        for (int j = 0; j < b; ++j)
+            for (int r = 0; r < ml; ++r) Sb[(size_t)j * m + idx[r]] = Sl[(size_t)j * ml + r];
+        // X = V S_b, residuals R = W S_b - X theta, corrections T and the
+        // partial sums of |R|^2 (downloaded with the next G columns)
+        std::copy(Sb.begin(), Sb.end(), pS);
+        std::copy(theta.begin(), theta.begin() + b, pS + (size_t)m * b);
+        DV_CHECK(hipMemcpyAsync(dv.S, pS, ((size_t)m * b + b) * 8, hipMemcpyHostToDevice, ctx->stream), "S");
+        DV_CHECK(dv.ritz(m), "ritz");
+        thetaPrev = theta;
+        have_prev = true;
+        if (ml >= n) {
+            // the subspace is the whole space: the Ritz pairs are exact
+            DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
             converged = true;
             break;
         }
@@ -530,7 +773,7 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
             // thick restart (GD+k; Stathopoulos & Saad 1998): keep the current
             // Ritz vectors and the previous iteration's, orthonormalised in
             // the m-dimensional coefficient space, so V <- V C, W <- W C
-            std::vector<double> C((size_t)m * kMaxB, 0.0);
+            std::vector<double> C((size_t)m * 2 * kMaxB, 0.0);
             std::copy(Sb.begin(), Sb.begin() + (size_t)m * b, C.begin());
             int keep = b;
             const int kprev = sp_rows > 0 ? std::min(b, kMaxB - b) : 0;
@@ -551,84 +794,42 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
                 for (int r = 0; r < m; ++r) C[(size_t)keep * m + r] = x[r] / nr;
                 ++keep;
             }
-            DV_CHECK(hipMemcpyAsync(dv.S, C.data(), (size_t)m * keep * 8, hipMemcpyHostToDevice, ctx->stream),
-                     "C");
-            DV_CHECK(dv.nn(dv.V, m, dv.S, m, keep, 1.0, 0.0, dv.AT), "VC");
+            std::copy(C.begin(), C.begin() + (size_t)m * keep, pC);
+            DV_CHECK(hipMemcpyAsync(dv.SC, pC, (size_t)m * keep * 8, hipMemcpyHostToDevice, ctx->stream), "C");
+            DV_CHECK(dv.nn(dv.V, m, nullptr, 0, dv.SC, m, keep, dv.AT), "VC");
             DV_CHECK(hipMemcpyAsync(dv.V, dv.AT, nn_ * keep * 8, hipMemcpyDeviceToDevice, ctx->stream), "V");
-            DV_CHECK(dv.nn(dv.W, m, dv.S, m, keep, 1.0, 0.0, dv.AT), "WC");
+            DV_CHECK(dv.nn(dv.W, m, nullptr, 0, dv.SC, m, keep, dv.AT), "WC");
             DV_CHECK(hipMemcpyAsync(dv.W, dv.AT, nn_ * keep * 8, hipMemcpyDeviceToDevice, ctx->stream), "W");
             m = keep;
             m0 = 0;
             sp_rows = 0;
+            std::fill(live.begin(), live.end(), 1);
         } else {
             Sp = Sb;
             sp_rows = m;
+            m0 = m;
         }
-        const bool restarted = sp_rows == 0;
-        // orthogonalise T against V twice: T -= V (V^T T)
-        for (int pass = 0; pass < 2; ++pass) {
-            DV_CHECK(dv.tn(dv.V, m, dv.T, b, dv.G, m), "VtT");
-            DV_CHECK(dv.nn(dv.V, m, dv.G, m, b, -1.0, 1.0, dv.T), "T-=VC");
-        }
-        // CholQR2 within T (drop to fewer columns if rank deficient); the
-        // second pass writes the new block straight into V's next columns
-        int nb = b;
-        {
-            const double *src = dv.T;
-            for (int pass = 0; pass < 2; ++pass) {
-                double *dst = pass == 0 ? dv.AT : dv.V + (size_t)m * n;
-                DV_CHECK(dv.tn(src, nb, src, nb, dv.G, nb), "TtT");
-                std::vector<double> U((size_t)nb * nb);
-                DV_CHECK(hipMemcpyAsync(U.data(), dv.G, U.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "TtT");
-                DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
-                if (!cholesky_upper(nb, U)) {
-                    nb = 0;
-                    break;
-                }
-                std::vector<double> Ui = upper_inverse(nb, U);
-                DV_CHECK(hipMemcpyAsync(dv.S, Ui.data(), Ui.size() * 8, hipMemcpyHostToDevice, ctx->stream), "Ui");
-                DV_CHECK(dv.nn(src, nb, dv.S, nb, nb, 1.0, 0.0, dst), "T U^-1");
-                src = dst;
-            }
-        }
-        if (nb == 0) {
-            // correction collapsed: fall back to the (orthogonalised) residuals
-            DV_CHECK(hipMemcpyAsync(dv.T, dv.R, nn_ * b * 8, hipMemcpyDeviceToDevice, ctx->stream), "T=R");
-            for (int pass = 0; pass < 2; ++pass) {
-                DV_CHECK(dv.tn(dv.V, m, dv.T, b, dv.G, m), "VtR");
-                DV_CHECK(dv.nn(dv.V, m, dv.G, m, b, -1.0, 1.0, dv.T), "R-=VC");
-            }
-            nb = b;
-            DV_CHECK(dv.tn(dv.T, nb, dv.T, nb, dv.G, nb), "RtR2");
-            std::vector<double> U((size_t)nb * nb);
-            DV_CHECK(hipMemcpyAsync(U.data(), dv.G, U.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "RtR2");
-            DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
-            if (!cholesky_upper(nb, U)) {
-                rc = fail(ctx, -8, "Davidson: search space collapsed");
-                cleanup();
-                return rc;
-            }
-            std::vector<double> Ui = upper_inverse(nb, U);
-            DV_CHECK(hipMemcpyAsync(dv.S, Ui.data(), Ui.size() * 8, hipMemcpyHostToDevice, ctx->stream), "Ui");
-            DV_CHECK(dv.nn(dv.T, nb, dv.S, nb, nb, 1.0, 0.0, dv.V + (size_t)m * n), "R U^-1");
-        }
-        DV_CHECK(dv.symm(dv.V + (size_t)m * n, nb, dv.W + (size_t)m * n), "AT");
-        m0 = restarted ? 0 : m;
-        m += nb;
+        // correction block: orthonormalised against V and within itself,
+        // written into V's next b columns, then its products with A
+        DV_CHECK(dv.orth_cholqr(m, b, dv.V + (size_t)m * n), "T orth");
+        DV_CHECK(dv.symm(dv.V + (size_t)m * n, b, dv.W + (size_t)m * n), "AT");
+        m += b;
+        DV_CHECK(dv.tn(dv.V, m, nullptr, 0, dv.W + (size_t)m0 * n, m - m0, dv.G, m), "VtW");
+        DV_CHECK(hipMemcpyAsync(pRR, dv.rr, (nrr + (size_t)m * (m - m0)) * 8, hipMemcpyDeviceToHost, ctx->stream),
+                 "RR, G");
     }
     if (!converged) {
         rc = fail(ctx, -9, "Davidson did not converge");
-        cleanup();
+        hipStreamSynchronize(ctx->stream);
         return rc;
     }
-    for (int j = 0; j < k; ++j) evals[j] = theta[j];
+    for (int j = 0; j < k; ++j) evals[j] = thetaPrev[j];
     if (evecs) {
-        std::vector<double> Xh(nn_ * b);
+        std::vector<double> Xh(nn_ * k);
         DV_CHECK(hipMemcpyAsync(Xh.data(), dv.X, Xh.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "X");
         DV_CHECK(hipStreamSynchronize(ctx->stream), "sync");
-        std::copy(Xh.begin(), Xh.begin() + nn_ * k, evecs);
+        std::copy(Xh.begin(), Xh.end(), evecs);
     }
-    cleanup();
 #undef DV_CHECK
     return 0;
 }

@@ -95,6 +95,12 @@ struct scp3g_ctx {
     double norm_sum = 0.0;
     // MLP surrogate state (mlp.hip), owned by the context
     struct MlpState *mlp = nullptr;
+    // Davidson work space, kept between solves (eigen.hip): device buffers
+    // and pinned host staging (hipHostMalloc)
+    double *d_dav = nullptr;
+    size_t dav_bytes = 0;
+    double *h_dav = nullptr;
+    size_t hdav_bytes = 0;
 
     // timing
     bool timing = false;
@@ -153,6 +159,7 @@ void release_tiles128(scp3g_ctx *ctx);
 double qtip4pf_energy_host(int nw, const double *crd);
 double ttm3f_energy_host(int nw, const double *crd, int *iters);
 void mlp_release(scp3g_ctx *ctx);
+void release_davidson(scp3g_ctx *ctx);
 
 // host helpers
 std::vector<double> ho_diagonal(const scp3g_ctx *ctx);  // HO diagonal + unselected ZPE

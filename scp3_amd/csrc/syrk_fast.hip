@@ -6,8 +6,9 @@
 //
 // With s_p = sign(w_p) and x_p = sqrt|w_p| b_p,  B diag(w) B^T = sum_p s_p
 // x_p x_p^T, so one operand array serves both sides of the product and the
-// sign is applied to the left operand as it leaves LDS (an XOR of the BF16
-// sign bits).  x is FP32 and is split exactly into hi = bf16(x),
+// sign is applied to the right operand as it leaves LDS (an XOR of the BF16
+// sign bits; the right operand is the smaller per-wave fragment, and the
+// left one then feeds the MFMAs without a VALU step in between).  x is FP32 and is split exactly into hi = bf16(x),
 // lo = bf16(x - hi); the product is hi.hi + hi.lo + lo.hi (relative error
 // ~2^-16), accumulated in FP32 by v_mfma_f32_16x16x32_bf16.
 //
@@ -450,14 +451,8 @@ __global__ void __launch_bounds__(512, 1)
         for (int a = 0; a < 8; ++a) {
             const int r = wr * 128 + 16 * a + li;
             const int p = r * 4 + (lg ^ ((r >> 1) & 3));
-            uint4 vh = lds[s][0][p];
-            vh.x ^= fl.x; vh.y ^= fl.y; vh.z ^= fl.z; vh.w ^= fl.w;
-            ah[a] = __builtin_bit_cast(bf16x8, vh);
-            if (TERMS == 3) {
-                uint4 vl = lds[s][1][p];
-                vl.x ^= fl.x; vl.y ^= fl.y; vl.z ^= fl.z; vl.w ^= fl.w;
-                al[a] = __builtin_bit_cast(bf16x8, vl);
-            }
+            ah[a] = __builtin_bit_cast(bf16x8, lds[s][0][p]);
+            if (TERMS == 3) al[a] = __builtin_bit_cast(bf16x8, lds[s][1][p]);
         }
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
@@ -471,8 +466,14 @@ __global__ void __launch_bounds__(512, 1)
             for (int bb = 0; bb < 2; ++bb) {
                 const int r = wc * 64 + 16 * (2 * half + bb) + li;
                 const int p = r * 4 + (lg ^ ((r >> 1) & 3));
-                bh[bb] = __builtin_bit_cast(bf16x8, lds[s][2][p]);
-                if (TERMS == 3) bl[bb] = __builtin_bit_cast(bf16x8, lds[s][3][p]);
+                uint4 vh = lds[s][2][p];
+                vh.x ^= fl.x; vh.y ^= fl.y; vh.z ^= fl.z; vh.w ^= fl.w;
+                bh[bb] = __builtin_bit_cast(bf16x8, vh);
+                if (TERMS == 3) {
+                    uint4 vl = lds[s][3][p];
+                    vl.x ^= fl.x; vl.y ^= fl.y; vl.z ^= fl.z; vl.w ^= fl.w;
+                    bl[bb] = __builtin_bit_cast(bf16x8, vl);
+                }
             }
             // term-major order: 16 independent accumulators between two
             // updates of the same one
