@@ -275,7 +275,7 @@ def test_mixed_kernels_agree(ctx, hexamer, n2, n3):
         ctx.reset()
         ctx.accumulate(1 << 30, P)
         out[name] = ctx.h_download()
-    ctx.set_mixed_kernel(2)
+    ctx.set_mixed_kernel(1)
     scale = np.abs(out["fp64"]).max()
     for k in ("fast", "m32"):
         d_fs = np.abs(out[k] - out["slow"]).max() / scale
@@ -283,6 +283,36 @@ def test_mixed_kernels_agree(ctx, hexamer, n2, n3):
         print("n=%d %s: |-slow| %.2e  |-fp64| %.2e (relative to max|H|)" % (out["fp64"].shape[0], k, d_fs, d_f64))
         assert d_fs < 2e-5 and d_f64 < 2e-5
         assert np.all(np.tril(out[k], -1) == 0)
+
+
+def test_w20_ttm3f_mixed_matches_fp64(ctx):
+    """20-water cluster under TTM3-F, all 174 modes selected (the mixed basis
+    kernel then stages 174 modes of Hermite values, 134 KiB of LDS), doubles
+    over the first 20 modes (385 states): the mixed form's H and lowest
+    levels against the FP64 form on the same points, and the direct-load
+    mixed SYRK against the LDS-tiled one."""
+    from scp3_amd import system
+    w20 = system.prepare(system.input_path("w20_ttm3f"), nmodes2=20, nmodes3=0)
+    P = 3 * 4096 + 77
+    out = {}
+    for name, mode, fast in (("fp64", scp3_amd.FP64, 1), ("slow", scp3_amd.MIXED, 0), ("fast", scp3_amd.MIXED, 1)):
+        print("w20 case", name, flush=True)
+        setup_system(ctx, w20, pes="ttm3f", precision=mode)
+        ctx.set_mixed_kernel(fast)
+        ctx.reset()
+        ctx.accumulate(1 << 30, P)
+        out[name] = (ctx.h_download(), ctx.lowest_levels(5))
+    ctx.set_mixed_kernel(1)
+    H64, ev64 = out["fp64"]
+    assert H64.shape == (385, 385)
+    scale = np.abs(H64).max()
+    for k in ("slow", "fast"):
+        H, ev = out[k]
+        dH = np.abs(H - H64).max() / scale
+        dlev = np.abs(ev - ev64).max() * scp3_amd.AUTOCM
+        print("w20 %s vs fp64: max|dH|/max|H| %.2e, max|dlevel| %.4f cm^-1" % (k, dH, dlev))
+        assert dH < 3e-4 and dlev < 0.5
+    assert np.abs(out["fast"][0] - out["slow"][0]).max() / scale < 2e-5
 
 
 def test_cpu_potential_path_matches_device_path(ctx, dimer):
