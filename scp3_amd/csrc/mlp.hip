@@ -48,13 +48,21 @@ enum { EPI_FWD = 0, EPI_BWD = 1, EPI_WGRAD = 2 };
 
 // C = A B^T on a 64 x 64 tile per workgroup (2 x 2 waves of 32 x 32, 2 x 2
 // blocks of 16 x 16 x 32 each).  M, N multiples of 64, K multiple of 32.
+// The weight-gradient form splits K over blockIdx.z: split z contracts
+// k in [z K, (z + 1) K) and writes its partial sums at G + z gsplit.
 template <int EPI>
 __global__ void __launch_bounds__(256)
     k_gemm_nt(const unsigned short *__restrict__ A, int64_t lda, const unsigned short *__restrict__ B,
               int64_t ldb, int K, const float *__restrict__ bias, float *__restrict__ Z,
               unsigned short *__restrict__ Out, unsigned short *__restrict__ OutT, int64_t ldo,
-              int64_t ldoT, const float *__restrict__ Zprev, float *__restrict__ G, int64_t ldg)
+              int64_t ldoT, const float *__restrict__ Zprev, float *__restrict__ G, int64_t ldg,
+              int64_t gsplit)
 {
+    if (EPI == EPI_WGRAD) {
+        A += (size_t)blockIdx.z * K;
+        B += (size_t)blockIdx.z * K;
+        G += (size_t)blockIdx.z * gsplit;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lg = lane >> 4;
     const int64_t m0 = (int64_t)blockIdx.y * 64 + (wave >> 1) * 32;
@@ -211,6 +219,16 @@ __global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float
     P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// G[i] = sum over splits s of part[s][i], in split order.
+__global__ void k_grad_reduce(const float *__restrict__ part, int nsplit, int64_t n, float *__restrict__ G)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float g = part[i];
+    for (int sp = 1; sp < nsplit; ++sp) g += part[(size_t)sp * n + i];
+    G[i] = g;
+}
+
 // bf16 copies W[o][i] and W^T[i][o] of an FP32 [o][i] matrix.
 __global__ void k_to_bf16(const float *__restrict__ W, int rows, int cols,
                           unsigned short *__restrict__ Wb, unsigned short *__restrict__ WbT)
@@ -295,6 +313,11 @@ struct MlpState {
     scp3g::Layer L[3];
     int64_t w3_off = 0, b3_off = 0, nparam = 0;
     float *P = nullptr, *G = nullptr, *m = nullptr, *v = nullptr;  // device
+    // split-K partial gradients [kMaxGradSplit][nparam]; entries that only
+    // split 0 writes (biases, head) stay zero in the other splits
+    float *Gpart = nullptr;
+    int gsplit = 1;           // splits summed by k_grad_reduce
+    int lsplit[3] = {1, 1, 1};  // splits of each layer's last weight gradient
     int64_t step = 0;
     double V0 = 0.0, t_mean = 0.0, t_std = 1.0;
     // work buffers for a batch of `cap` points
@@ -323,6 +346,8 @@ static void mfree(T *&p)
     mfree(q);
     p = nullptr;
 }
+
+constexpr int kMaxGradSplit = 16;
 
 static void free_batch(MlpState *s)
 {
@@ -353,6 +378,7 @@ void mlp_release(scp3g_ctx *ctx)
     }
     mfree(s->P);
     mfree(s->G);
+    mfree(s->Gpart);
     mfree(s->m);
     mfree(s->v);
     mfree(s->Xtr);
@@ -412,7 +438,7 @@ static hipError_t forward(scp3g_ctx *ctx, MlpState *s, const unsigned short *x, 
         hipLaunchKernelGGL(k_gemm_nt<EPI_FWD>, grid, dim3(256), 0, ctx->stream, in, ldin, Ly.Wb,
                            (int64_t)Ly.in, Ly.in, s->P + Ly.b_off, s->z[l], s->act[l],
                            keepT ? s->actT[l] : (unsigned short *)nullptr, (int64_t)s->h,
-                           (int64_t)s->cap, (const float *)nullptr, (float *)nullptr, (int64_t)0);
+                           (int64_t)s->cap, (const float *)nullptr, (float *)nullptr, (int64_t)0, (int64_t)0);
         in = s->act[l];
         ldin = s->h;
     }
@@ -481,29 +507,44 @@ static hipError_t grad_step(scp3g_ctx *ctx, MlpState *s, int64_t b0, int64_t B, 
     hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, ctx->stream, s->sq, B, lossd);
     // head: w3, b3
     hipLaunchKernelGGL(k_head_grad, dim3((unsigned)(s->h + 1)), dim3(256), 0, ctx->stream,
-                       s->actT[2], (int64_t)s->cap, s->dout, B, s->G + s->w3_off, s->G + s->b3_off);
+                       s->actT[2], (int64_t)s->cap, s->dout, B, s->Gpart + s->w3_off, s->Gpart + s->b3_off);
     for (int l = 2; l >= 0; --l) {
         const Layer &Ly = s->L[l];
         // bias gradient: row sums of delta^T
         hipLaunchKernelGGL(k_rowsum_bf16, dim3((unsigned)Ly.out), dim3(256), 0, ctx->stream,
-                           s->dlT[l], (int64_t)s->cap, B, s->G + Ly.b_off);
-        // weight gradient G[o][i] = sum_p delta[p][o] a_in[p][i]
+                           s->dlT[l], (int64_t)s->cap, B, s->Gpart + Ly.b_off);
+        // weight gradient G[o][i] = sum_p delta[p][o] a_in[p][i], the batch
+        // split so that the grid has >= 256 workgroups
         const unsigned short *inT = l > 0 ? s->actT[l - 1] : xT;
         const int64_t ldinT = l > 0 ? s->cap : s->ntrain;
-        hipLaunchKernelGGL(k_gemm_nt<EPI_WGRAD>, dim3((unsigned)(Ly.in / 64), (unsigned)(Ly.out / 64)),
-                           dim3(256), 0, ctx->stream, s->dlT[l], (int64_t)s->cap, inT, ldinT, (int)B,
+        const int ntile = (Ly.in / 64) * (Ly.out / 64);
+        int nsplit = 1;
+        while (nsplit < kMaxGradSplit && ntile * nsplit < 256 && (B / (2 * nsplit)) % 32 == 0) nsplit *= 2;
+        hipLaunchKernelGGL(k_gemm_nt<EPI_WGRAD>,
+                           dim3((unsigned)(Ly.in / 64), (unsigned)(Ly.out / 64), (unsigned)nsplit),
+                           dim3(256), 0, ctx->stream, s->dlT[l], (int64_t)s->cap, inT, ldinT, (int)(B / nsplit),
                            (const float *)nullptr, (float *)nullptr, (unsigned short *)nullptr,
                            (unsigned short *)nullptr, (int64_t)0, (int64_t)0, (const float *)nullptr,
-                           s->G + Ly.w_off, (int64_t)Ly.in);
+                           s->Gpart + Ly.w_off, (int64_t)Ly.in, s->nparam);
+        if (nsplit < s->lsplit[l]) {
+            // a smaller batch than before: clear this layer's stale partials
+            for (int sp = nsplit; sp < s->lsplit[l]; ++sp)
+                hipMemsetAsync(s->Gpart + (size_t)sp * s->nparam + Ly.w_off, 0, (size_t)Ly.out * Ly.in * 4,
+                               ctx->stream);
+        }
+        s->lsplit[l] = nsplit;
+        if (nsplit > s->gsplit) s->gsplit = nsplit;
         if (l > 0) {
             // delta_{l-1}[p][i] = sum_o delta_l[p][o] W[o][i] * silu'(z_{l-1}[p][i])
             hipLaunchKernelGGL(k_gemm_nt<EPI_BWD>, dim3((unsigned)(Ly.in / 64), (unsigned)(B / 64)),
                                dim3(256), 0, ctx->stream, s->dl[l], (int64_t)s->h, Ly.WbT,
                                (int64_t)Ly.out, Ly.out, (const float *)nullptr, (float *)nullptr,
                                s->dl[l - 1], s->dlT[l - 1], (int64_t)s->h, (int64_t)s->cap,
-                               s->z[l - 1], (float *)nullptr, (int64_t)0);
+                               s->z[l - 1], (float *)nullptr, (int64_t)0, (int64_t)0);
         }
     }
+    hipLaunchKernelGGL(k_grad_reduce, dim3((unsigned)((s->nparam + 255) / 256)), dim3(256), 0, ctx->stream,
+                       s->Gpart, s->gsplit, s->nparam, s->G);
     return hipGetLastError();
 }
 
@@ -518,6 +559,27 @@ static uint64_t splitmix64(uint64_t &x)
 }  // namespace scp3g
 
 using namespace scp3g;
+
+// Training set from device inputs y (M x npts, mode-major, ld npts) and host
+// targets t: BF16 inputs in both layouts, targets on the device.
+static int load_training_set(scp3g_ctx *ctx, MlpState *s, const double *d_y, const float *t_host, int64_t npts)
+{
+    mfree(s->Xtr);
+    mfree(s->XtrT);
+    mfree(s->Ttr);
+    s->ntrain = 0;
+    hipError_t e = hipMalloc((void **)&s->Xtr, (size_t)npts * s->dpad * 2);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->XtrT, (size_t)npts * s->dpad * 2);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->Ttr, (size_t)npts * 4);
+    int rc = check_hip(ctx, e, "alloc training set");
+    if (rc) return rc;
+    e = hipMemcpyAsync(s->Ttr, t_host, (size_t)npts * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = inputs(ctx, s, d_y, npts, npts, s->Xtr, s->XtrT, npts);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if ((rc = check_hip(ctx, e, "inputs"))) return rc;
+    s->ntrain = npts;
+    return 0;
+}
 
 extern "C" {
 
@@ -567,6 +629,7 @@ int scp3g_mlp_init(scp3g_ctx *ctx, int hidden, uint64_t seed)
     hipError_t e = hipSuccess;
     for (float **p : {&s->P, &s->G, &s->m, &s->v})
         if (e == hipSuccess) e = hipMalloc((void **)p, (size_t)off * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->Gpart, (size_t)kMaxGradSplit * off * 4);
     for (int l = 0; l < 3; ++l) {
         if (e == hipSuccess) e = hipMalloc((void **)&s->L[l].Wb, (size_t)hidden * ins[l] * 2);
         if (e == hipSuccess) e = hipMalloc((void **)&s->L[l].WbT, (size_t)hidden * ins[l] * 2);
@@ -574,6 +637,7 @@ int scp3g_mlp_init(scp3g_ctx *ctx, int hidden, uint64_t seed)
     if ((rc = check_hip(ctx, e, "alloc mlp"))) return rc;
     hipMemcpy(s->P, P.data(), P.size() * 4, hipMemcpyHostToDevice);
     hipMemset(s->G, 0, P.size() * 4);
+    hipMemset(s->Gpart, 0, (size_t)kMaxGradSplit * P.size() * 4);
     hipMemset(s->m, 0, P.size() * 4);
     hipMemset(s->v, 0, P.size() * 4);
     // V0: the potential at the reference geometry, hartree (FP64 CPU)
@@ -629,24 +693,14 @@ int scp3g_mlp_load_data(scp3g_ctx *ctx, const double *y_host, const float *t_hos
     if (rc) return rc;
     MlpState *s = ctx->mlp;
     if (npts <= 0 || npts % 64) return fail(ctx, -4, "training points must be a positive multiple of 64");
-    mfree(s->Xtr);
-    mfree(s->XtrT);
-    mfree(s->Ttr);
-    hipError_t e = hipMalloc((void **)&s->Xtr, (size_t)npts * s->dpad * 2);
-    if (e == hipSuccess) e = hipMalloc((void **)&s->XtrT, (size_t)npts * s->dpad * 2);
-    if (e == hipSuccess) e = hipMalloc((void **)&s->Ttr, (size_t)npts * 4);
-    if ((rc = check_hip(ctx, e, "alloc training set"))) return rc;
     double *d_y = nullptr;
     if ((rc = check_hip(ctx, hipMalloc(&d_y, (size_t)npts * s->M * 8), "alloc y"))) return rc;
     std::vector<double> ym((size_t)npts * s->M);
     for (int64_t p = 0; p < npts; ++p)
         for (int k = 0; k < s->M; ++k) ym[(size_t)k * npts + p] = y_host[(size_t)p * s->M + k];
-    hipMemcpy(d_y, ym.data(), ym.size() * 8, hipMemcpyHostToDevice);
-    hipMemcpy(s->Ttr, t_host, (size_t)npts * 4, hipMemcpyHostToDevice);
-    inputs(ctx, s, d_y, npts, npts, s->Xtr, s->XtrT, npts);
-    rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "inputs");
+    rc = check_hip(ctx, hipMemcpy(d_y, ym.data(), ym.size() * 8, hipMemcpyHostToDevice), "upload y");
+    if (!rc) rc = load_training_set(ctx, s, d_y, t_host, npts);
     hipFree(d_y);
-    s->ntrain = npts;
     return rc;
 }
 
@@ -659,40 +713,48 @@ int scp3g_mlp_sample_data(scp3g_ctx *ctx, int64_t skip, int64_t npts)
     if (rc) return rc;
     MlpState *s = ctx->mlp;
     if (npts <= 0 || npts % 64) return fail(ctx, -4, "training points must be a positive multiple of 64");
+    hipSetDevice(ctx->device);
+    // the whole set stays on the device: y (mode-major, ld npts) in tiles of
+    // T points through Sobol -> map -> potential (FP64) -> targets; only the
+    // targets visit the host, for their mean and scale in FP64
     const int M = s->M;
-    std::vector<double> yh((size_t)npts * M), raw(npts);
-    const int64_t T = 4096;
+    const int64_t T = std::min<int64_t>(npts, 65536);
     double *d_y = nullptr, *d_crd = nullptr, *d_E = nullptr, *d_raw = nullptr;
-    hipError_t e = hipMalloc(&d_y, (size_t)M * T * 8);
+    hipError_t e = hipMalloc(&d_y, (size_t)M * npts * 8);
     if (e == hipSuccess) e = hipMalloc(&d_crd, (size_t)9 * ctx->nw * T * 8);
     if (e == hipSuccess) e = hipMalloc(&d_E, (size_t)T * 8);
-    if (e == hipSuccess) e = hipMalloc(&d_raw, (size_t)T * 8);
-    if ((rc = check_hip(ctx, e, "alloc sampling"))) return rc;
-    std::vector<double> yt((size_t)M * T);
-    for (int64_t done = 0; done < npts && !rc; done += T) {
+    if (e == hipSuccess) e = hipMalloc(&d_raw, (size_t)npts * 8);
+    auto release = [&]() {
+        hipStreamSynchronize(ctx->stream);
+        hipFree(d_y);
+        hipFree(d_crd);
+        hipFree(d_E);
+        hipFree(d_raw);
+    };
+    if ((rc = check_hip(ctx, e, "alloc sampling"))) {
+        release();
+        return rc;
+    }
+    for (int64_t done = 0; done < npts && e == hipSuccess; done += T) {
         const int64_t np = std::min<int64_t>(T, npts - done);
-        e = launch_sobol(ctx, ctx->d_sobol, M, skip + done, np, 1, d_y, T, nullptr);
-        if (e == hipSuccess) e = launch_map(ctx, d_y, T, np, d_crd, T, 0);
+        e = launch_sobol(ctx, ctx->d_sobol, M, skip + done, np, 1, d_y + done, npts, nullptr);
+        if (e == hipSuccess) e = launch_map(ctx, d_y + done, npts, np, d_crd, T, 0);
         if (e == hipSuccess)
             e = ctx->pes == SCP3G_PES_TTM3F ? launch_ttm3f(ctx, ctx->nw, np, d_crd, T, 0, d_E)
                                             : launch_qtip4pf(ctx, ctx->nw, np, d_crd, T, 0, d_E);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_mlp_targets, dim3((unsigned)((np + 255) / 256)), dim3(256), 0,
-                               ctx->stream, d_E, d_y, T, ctx->d_omega, M, s->V0, np, d_raw);
+                               ctx->stream, d_E, d_y + done, npts, ctx->d_omega, M, s->V0, np, d_raw + done);
             e = hipGetLastError();
         }
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e == hipSuccess) e = hipMemcpy(yt.data(), d_y, yt.size() * 8, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(raw.data() + done, d_raw, np * 8, hipMemcpyDeviceToHost);
-        rc = check_hip(ctx, e, "sample training set");
-        for (int64_t p = 0; p < np; ++p)
-            for (int k = 0; k < M; ++k) yh[(size_t)(done + p) * M + k] = yt[(size_t)k * T + p];
     }
-    hipFree(d_y);
-    hipFree(d_crd);
-    hipFree(d_E);
-    hipFree(d_raw);
-    if (rc) return rc;
+    std::vector<double> raw(npts);
+    if (e == hipSuccess) e = hipMemcpyAsync(raw.data(), d_raw, (size_t)npts * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if ((rc = check_hip(ctx, e, "sample training set"))) {
+        release();
+        return rc;
+    }
     double mean = 0.0, var = 0.0;
     for (double r : raw) mean += r;
     mean /= (double)npts;
@@ -702,7 +764,9 @@ int scp3g_mlp_sample_data(scp3g_ctx *ctx, int64_t skip, int64_t npts)
     s->t_std = var > 0 ? std::sqrt(var) : 1.0;
     std::vector<float> t(npts);
     for (int64_t p = 0; p < npts; ++p) t[p] = (float)((raw[p] - mean) / s->t_std);
-    return scp3g_mlp_load_data(ctx, yh.data(), t.data(), npts);
+    rc = load_training_set(ctx, s, d_y, t.data(), npts);
+    release();
+    return rc;
 }
 
 // Adam on batches of `batch` consecutive training points (cycling through
