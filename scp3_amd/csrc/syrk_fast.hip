@@ -406,37 +406,50 @@ __global__ void __launch_bounds__(512, 1)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 
-    // Operands move global -> LDS by DMA (global_load_lds_dwordx4), one step
-    // ahead, in 1 KiB blocks of 16 rows of one plane: 64 blocks per step for
-    // TERMS = 3 (8 per wave), 32 for TERMS = 1 (4 per wave, hi planes only).
-    // Lane l of a block writes LDS position 16 l, i.e. row l >> 2, slot l & 3,
-    // which holds chunk (l & 3) ^ ((row >> 1) & 3) of that row; so each lane
-    // fetches that chunk, and a block is still one contiguous KiB of global
-    // memory.
-    constexpr int NBLK = TERMS == 3 ? 8 : 4;
-    const uint4 *gsrc[NBLK];
-    int loff[NBLK];
+    // Operands move global -> LDS by DMA (global_load_lds_dwordx4), one stage
+    // ahead, in 1 KiB blocks of 16 rows of one plane slot, 64 blocks (8 per
+    // wave) per stage.  TERMS = 3: a stage is one 32-point k-block, slots
+    // A-hi, A-lo, B-hi, B-lo.  TERMS = 1 needs only the hi planes, so a stage
+    // holds two k-blocks, slots A-hi[kb], A-hi[kb + 1], B-hi[kb], B-hi[kb + 1]
+    // (half the barriers per MFMA; 9% faster at 20825 states).  Lane l of a
+    // block writes LDS position 16 l, i.e. row l >> 2, slot l & 3, which holds
+    // chunk (l & 3) ^ ((row >> 1) & 3) of that row; so each lane fetches that
+    // chunk, and a block is still one contiguous KiB of global memory.
+    constexpr int KS = TERMS == 3 ? 1 : 2;  // k-blocks per stage
+    const uint4 *gsrc[8];
+    int loff[8];
 #pragma unroll
-    for (int i = 0; i < NBLK; ++i) {
-        const int blk = wave * NBLK + i;
-        const int plane = TERMS == 3 ? blk >> 4 : 2 * (blk >> 4);  // 0 Ah, 1 Al, 2 Bh, 3 Bl
+    for (int i = 0; i < 8; ++i) {
+        const int blk = wave * 8 + i;
+        const int slot = blk >> 4;
         const int rb = blk & 15;
         const int r = rb * 16 + (lane >> 2);
         const int c = (lane & 3) ^ ((r >> 1) & 3);
-        gsrc[i] = ((plane & 1) ? Xl : Xh) + ((plane < 2 ? rowA : rowB) + r) * 4 + c;
-        loff[i] = plane * (kFT * 4) + rb * 64;
+        const size_t row = (slot < 2 ? rowA : rowB) + r;
+        gsrc[i] = TERMS == 3 ? ((slot & 1) ? Xl : Xh) + row * 4 + c : Xh + row * 4 + c;
+        loff[i] = slot * (kFT * 4) + rb * 64;
     }
+    // DMA of the stage that starts at k-block kbv (the second k-block of a
+    // TERMS = 1 stage is clamped to the last one of the range; it is then
+    // loaded but not used)
 #define X3_DMA(arr, kbv)                                                                          \
     do {                                                                                          \
-        const size_t kofs_ = (size_t)(kbv) * planeStride;                                         \
-        _Pragma("unroll") for (int i = 0; i < NBLK; ++i)                                          \
-            __builtin_amdgcn_global_load_lds((const void *)(gsrc[i] + kofs_),                     \
+        const int k0_ = (kbv);                                                                    \
+        const int k1_ = k0_ + 1 < kb1 ? k0_ + 1 : k0_;                                            \
+        /* TERMS = 1: this wave's 8 blocks all lie in slot (wave >> 1); the odd  */              \
+        /* slots hold the stage's second k-block                                 */              \
+        const int kk_ = (TERMS == 1 && ((wave >> 1) & 1)) ? k1_ : k0_;                            \
+        _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                           \
+            __builtin_amdgcn_global_load_lds((const void *)(gsrc[i] + (size_t)kk_ * planeStride), \
                 (__attribute__((address_space(3))) void *)(&(arr)[0][0] + loff[i]), 16, 0, 0);    \
+        }                                                                                         \
     } while (0)
-    // One 32-point step from stage arr.  The sign flips of this lane's 8
-    // points (8 lg .. 8 lg + 7) go onto the B fragments; the MFMAs run term-
-    // major, 16 independent accumulators between two updates of one.
-#define X3_STEP(arr, kbv)                                                                         \
+    // One 32-point k-block kbv from stage arr, operand slots sa (A) and sb (B)
+    // (TERMS = 3: A-lo is slot sa + 1, B-lo slot sb + 1).  The sign flips of
+    // this lane's 8 points (8 lg .. 8 lg + 7) go onto the B fragments; the
+    // MFMAs run term-major, 16 independent accumulators between two updates
+    // of one.
+#define X3_KBLOCK(arr, kbv, sa, sb)                                                               \
     do {                                                                                          \
         const unsigned bits_ = (sgn[kbv] >> (8 * lg)) & 0xffu;                                    \
         uint4 fl_;                                                                                \
@@ -448,16 +461,16 @@ __global__ void __launch_bounds__(512, 1)
         _Pragma("unroll") for (int a = 0; a < 8; ++a) {                                           \
             const int r_ = wr * 128 + 16 * a + li;                                                \
             const int p_ = r_ * 4 + (lg ^ ((r_ >> 1) & 3));                                       \
-            ah_[a] = __builtin_bit_cast(bf16x8, (arr)[0][p_]);                                    \
-            if (TERMS == 3) al_[a] = __builtin_bit_cast(bf16x8, (arr)[1][p_]);                    \
+            ah_[a] = __builtin_bit_cast(bf16x8, (arr)[sa][p_]);                                   \
+            if (TERMS == 3) al_[a] = __builtin_bit_cast(bf16x8, (arr)[(sa) + 1][p_]);             \
         }                                                                                         \
         _Pragma("unroll") for (int half = 0; half < 2; ++half) {                                  \
             bf16x8 bh_[2], bl_[2];                                                                \
             _Pragma("unroll") for (int bb = 0; bb < 2; ++bb) {                                    \
                 const int r_ = wc * 64 + 16 * (2 * half + bb) + li;                               \
                 const int p_ = r_ * 4 + (lg ^ ((r_ >> 1) & 3));                                   \
-                bh_[bb] = __builtin_bit_cast(bf16x8, xor4((arr)[2][p_], fl_));                    \
-                if (TERMS == 3) bl_[bb] = __builtin_bit_cast(bf16x8, xor4((arr)[3][p_], fl_));    \
+                bh_[bb] = __builtin_bit_cast(bf16x8, xor4((arr)[sb][p_], fl_));                   \
+                if (TERMS == 3) bl_[bb] = __builtin_bit_cast(bf16x8, xor4((arr)[(sb) + 1][p_], fl_)); \
             }                                                                                     \
             _Pragma("unroll") for (int term = 0; term < TERMS; ++term)                            \
                 _Pragma("unroll") for (int a = 0; a < 8; ++a)                                     \
@@ -469,25 +482,38 @@ __global__ void __launch_bounds__(512, 1)
                     }                                                                             \
         }                                                                                         \
     } while (0)
+    // the k-blocks kbv.. of stage arr (cnt_ of them, at most KS)
+#define X3_STEP(arr, kbv, cnt_)                                                                   \
+    do {                                                                                          \
+        if (TERMS == 3) {                                                                         \
+            X3_KBLOCK(arr, kbv, 0, 2);                                                            \
+        } else {                                                                                  \
+            X3_KBLOCK(arr, kbv, 0, 2);                                                            \
+            if ((cnt_) > 1) X3_KBLOCK(arr, (kbv) + 1, 1, 3);                                      \
+        }                                                                                         \
+    } while (0)
     // each __syncthreads waits for this wave's DMA into the stage about to be
     // read (the compiler's vmcnt(0) before s_barrier) and, through the
     // barrier, for every wave's DMA into it and every wave's reads of the
     // other stage, which the next DMA then overwrites
     if (kb0 < kb1) X3_DMA(lds0, kb0);
     int kb = kb0;
-    for (; kb + 2 <= kb1; kb += 2) {
+    for (; kb + 2 * KS <= kb1; kb += 2 * KS) {
         __syncthreads();
-        X3_DMA(lds1, kb + 1);
-        X3_STEP(lds0, kb);
+        X3_DMA(lds1, kb + KS);
+        X3_STEP(lds0, kb, KS);
         __syncthreads();
-        X3_DMA(lds0, kb + 2 < kb1 ? kb + 2 : kb + 1);
-        X3_STEP(lds1, kb + 1);
+        X3_DMA(lds0, kb + 2 * KS < kb1 ? kb + 2 * KS : kb + KS);
+        X3_STEP(lds1, kb + KS, KS);
     }
+    // the launcher makes every k-range a multiple of KS, so at most one
+    // stage is left (already in lds0)
     if (kb < kb1) {
         __syncthreads();
-        X3_STEP(lds0, kb);
+        X3_STEP(lds0, kb, kb1 - kb);
     }
 #undef X3_DMA
+#undef X3_KBLOCK
 #undef X3_STEP
 
     // D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4
@@ -719,8 +745,10 @@ hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
     const int nkb = (int)(K / 32);
     int nsplit = 1;
     while (ntiles * nsplit < 512 && nkb / (nsplit * 2) >= 8) nsplit *= 2;
-    const int kbs = (nkb + nsplit - 1) / nsplit;
+    int kbs = (nkb + nsplit - 1) / nsplit;
+    if (terms == 1) kbs += kbs & 1;  // k-ranges of whole two-block stages
     nsplit = (nkb + kbs - 1) / kbs;
+    if (terms == 1 && (nkb & 1)) return hipErrorInvalidValue;  // K is a multiple of 64
     if (kbs > kMaxSignWords) return hipErrorInvalidValue;  // tile above 131072 points
     float *part = nullptr;
     if (nsplit > 1) {
