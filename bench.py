@@ -59,6 +59,7 @@ def parse():
     ap.add_argument("--preset", default="c2", choices=sorted(PRESETS))
     ap.add_argument("--points", type=int, default=None, help="log2 of Sobol points per step")
     ap.add_argument("--forms", default=None)
+    ap.add_argument("--tile", type=int, default=None, help="points per tile (0 = automatic)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline time budget")
     ap.add_argument("--cpu-chunk", type=int, default=8192, help="points per CPU baseline chunk")
     ap.add_argument("--no-cpu", action="store_true")
@@ -217,6 +218,8 @@ def main():
         pr["points"] = a.points
     if a.forms:
         pr["forms"] = a.forms
+    if a.tile is not None:
+        pr["tile"] = a.tile
     pr["mixed_kernel"] = a.mixed_kernel
     sysd = system.prepare(system.input_path(pr["config"]), nmodes2=pr["n2"], nmodes3=pr["n3"])
     sysd["name"] = pr["config"]

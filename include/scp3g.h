@@ -87,12 +87,13 @@ int scp3g_set_pes(scp3g_ctx *ctx, const char *name);
 /* SCP3G_FP64 or SCP3G_MIXED */
 int scp3g_set_precision(scp3g_ctx *ctx, int mode);
 /* SYRK kernels: 1 (default) = signed 256 x 256-tile split-BF16 kernel on
- * X = sqrt|w| B, 8 waves on 16x16x32 MFMA; 2 = the same with 4 waves on
- * 32x32x16 MFMA and LDS DMA; 0 = 64 x 64-tile direct-load kernels on B and
- * w.  Any nonzero value selects the LDS-tiled FP64 kernel in the FP64 form. */
+ * X = sqrt|w| B, 8 waves on 16x16x32 MFMA, operands moved to LDS by DMA;
+ * 2 = the same tile with 4 waves on 32x32x16 MFMA, register-staged;
+ * 0 = 64 x 64-tile direct-load kernels on B and w.  Any nonzero value
+ * selects the LDS-tiled FP64 kernel in the FP64 form. */
 int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast);
 /* points per device tile (multiple of 256), or 0 (default): the largest power
- * of two <= 131072 whose basis table fits in 8 GiB, at least 4096 */
+ * of two <= 524288 whose basis table fits in 8 GiB, at least 4096 */
 int scp3g_set_tile(scp3g_ctx *ctx, int tile);
 
 /* ---- whole-path entry points -------------------------------------- */

@@ -178,12 +178,13 @@ static void free_sums(scp3g_ctx *ctx)
 
 // Points per tile when the caller sets none: per-point kernels need >= 64k
 // points to fill 256 CUs and the split-K SYRK partials amortise over the
-// tile, so take the largest power of two <= 2^17 whose FP64 table fits in
-// 8 GiB (HBM is 288 GB), but at least 4096.
+// tile, so take the largest power of two <= 2^19 whose FP64 table fits in
+// 8 GiB (HBM is 288 GB), but at least 4096.  (1225 states, 2^20 points:
+// 116, 124, 128, 128 M points/s at 2^17, 2^18, 2^19, 2^20.)
 static int64_t auto_tile(int64_t n)
 {
     const double rows = (double)rows_pad128(n) + 32;
-    int64_t T = 1 << 17;
+    int64_t T = 1 << 19;
     while (T > 4096 && 8.0 * rows * (double)T > 8.0 * (1 << 30)) T >>= 1;
     return T;
 }

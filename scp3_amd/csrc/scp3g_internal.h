@@ -63,7 +63,8 @@ struct scp3g_ctx {
     // kernels (syrk.hip, syrk_mixed.hip)
     bool mixed_fast = true;
     // signed split-BF16 SYRK variant: 1 = 8 waves of 128 x 64 on 16x16x32
-    // MFMA, 2 = 4 waves of 128 x 128 on 32x32x16 MFMA with LDS DMA
+    // MFMA, operands to LDS by DMA; 2 = 4 waves of 128 x 128 on 32x32x16
+    // MFMA, register-staged
     int x3_variant = 1;
     // control variate, mixed form: the surrogate term with hi.hi products only
     bool cv_one_term = false;

@@ -749,7 +749,8 @@ hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
     if (terms == 1) kbs += kbs & 1;  // k-ranges of whole two-block stages
     nsplit = (nkb + kbs - 1) / kbs;
     if (terms == 1 && (nkb & 1)) return hipErrorInvalidValue;  // K is a multiple of 64
-    if (kbs > kMaxSignWords) return hipErrorInvalidValue;  // tile above 131072 points
+    // the 32x32 variant stages a split's sign words in LDS
+    if (ctx->x3_variant == 2 && kbs > kMaxSignWords) return hipErrorInvalidValue;
     float *part = nullptr;
     if (nsplit > 1) {
         const size_t need = (size_t)nsplit * ntiles * kFT * kFT * sizeof(float);
