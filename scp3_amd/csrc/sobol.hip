@@ -63,7 +63,9 @@ __device__ __forceinline__ double bsm(double u)
         x = yj * (((kBsmA[3] * r + kBsmA[2]) * r + kBsmA[1]) * r + kBsmA[0]) /
             ((((kBsmB[3] * r + kBsmB[2]) * r + kBsmB[1]) * r + kBsmB[0]) * r + 1);
     } else {
-        r = (yj > 0) ? log(-log(1 - u)) : log(-log(u));
+        // one pair of logs per lane: the operand is chosen first
+        const double q = (yj > 0) ? 1 - u : u;
+        r = log(-log(q));
         x = kBsmC[0] +
             r * (kBsmC[1] +
                  r * (kBsmC[2] +
