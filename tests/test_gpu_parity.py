@@ -285,6 +285,59 @@ def test_mixed_kernels_agree(ctx, hexamer, n2, n3):
         assert np.all(np.tril(out[k], -1) == 0)
 
 
+def test_max_basis_20825_states(ctx, hexamer):
+    """The largest basis of the workloads (total quanta <= 3 over 48 modes,
+    20825 states, 3.47 GB of FP64 H) on a ragged point count: FP64 H against
+    the oracle on sampled elements (all points, in chunks), the mixed form
+    against the FP64 form on the whole upper triangle, lower triangles zero."""
+    rng = np.random.default_rng(11)
+    P = (1 << 15) + 123
+    out = {}
+    for mode in (scp3_amd.FP64, scp3_amd.MIXED):
+        print("max-basis case precision", mode, flush=True)
+        ctx.set_system(hexamer["x0"], hexamer["omega"], hexamer["MUa"])
+        ctx.set_basis(hexamer["modes"], 48, 48)
+        ctx.set_pes("qtip4pf")
+        ctx.set_precision(mode)
+        ctx.set_tile(0)
+        ctx.reset()
+        ctx.accumulate(1 << 30, P)
+        out[mode] = ctx.h_download(raw=True)
+    H64, Hmx = out[scp3_amd.FP64], out[scp3_amd.MIXED]
+    n = H64.shape[0]
+    assert n == 20825
+    # sampled elements: random pairs, the diagonal and the last rows/columns
+    ii = rng.integers(0, n, 3000)
+    jj = rng.integers(0, n, 3000)
+    ii, jj = np.minimum(ii, jj), np.maximum(ii, jj)
+    d = rng.integers(0, n, 300)
+    ii = np.concatenate([ii, d, np.arange(n - 40, n), rng.integers(0, n, 40)])
+    jj = np.concatenate([jj, d, np.full(40, n - 1), np.arange(n - 40, n)])
+    ii, jj = np.minimum(ii, jj), np.maximum(ii, jj)
+    ref = np.zeros(ii.size)
+    skip, chunk = 1 << 30, 4096
+    for p0 in range(0, P, chunk):
+        m = min(chunk, P - p0)
+        y = oracle.sobol_normals(48, skip + p0, m)
+        r = scp3_amd.BOHR * (hexamer["x0"][None, :] + y @ hexamer["MUa"].T)
+        V = np.array([oracle.qtip4pf(r[k]) for k in range(m)]) / scp3_amd.AUTOKCALPMOL
+        bV = V - 0.5 * (y * hexamer["omega"] * y).sum(1)
+        bra = oracle.get_bra(y[:, hexamer["modes"]], 48, 48, 48)
+        ref += np.einsum("pk,p,pk->k", bra[:, ii], bV, bra[:, jj])
+        del bra
+    got = H64[ii, jj]  # upper triangle: row ii <= column jj
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max() / scale
+    print("max-basis fp64 vs oracle on %d elements: %.2e" % (ii.size, err))
+    assert err < 1e-9
+    scale = np.abs(H64).max()
+    dmx = np.abs(Hmx - H64).max() / scale
+    print("max-basis mixed vs fp64: max|dH|/max|H| %.2e" % dmx)
+    assert dmx < 2e-5
+    for H in (H64, Hmx):
+        assert not np.any(np.tril(H, -1))
+
+
 def test_w20_ttm3f_mixed_matches_fp64(ctx):
     """20-water cluster under TTM3-F, all 174 modes selected (the mixed basis
     kernel then stages 174 modes of Hermite values, 134 KiB of LDS), doubles
