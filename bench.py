@@ -2,9 +2,11 @@
 
 Default workload (preset c2): water hexamer relaxed under q-TIP4P/F (6 H2O,
 48 modes), basis of total quanta <= 2 (1225 states), 2^20 Sobol points
-(skip 2^30) in 131072-point tiles (the automatic tile).  One step = one full pass of the hot path
-over the 2^20 points (sample -> map -> potential -> basis -> SYRK into the
-FP64 running H) followed by the lowest 10 levels on the GPU.  Both GPU forms
+(skip 2^30) in 524288-point tiles (the automatic tile).  One step = one
+full pass of the hot path over the 2^20 points (sample -> map -> potential -> basis -> SYRK into the
+FP64 running H) followed by the lowest 10 levels on the GPU (the K timed
+accumulate passes run first, with per-kernel event timers, then K timed
+eigensolves of the last pass's H without them).  Both GPU forms
 run: the plain FP64 form and the mixed form (FP32 potential and basis,
 3-term split-BF16 MFMA SYRK folded into FP64 H).  `value` is the mixed
 form's points/s into H; the FP64 form, the time-to-levels of both and the
@@ -121,6 +123,10 @@ def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
         one()
         c.lowest_levels(k, LEVEL_TOL)
     c.sync()
+    # accumulate steps with the per-kernel event timers on (the roofline
+    # needs the SYRK kernel's own time), then the eigensolves on the last
+    # step's H with the timers off (their events would add host latency to
+    # a latency-bound solver)
     c.set_timing(True)
     t_acc = t_lev = 0.0
     ev = None
@@ -129,16 +135,17 @@ def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
         t0 = time.perf_counter()
         one()
         c.sync()
+        t_acc += time.perf_counter() - t0
+    kt = {}
+    for name in ("sobol", "map", "pes", "weight", "mlp", "basis", "syrk", "syrk_mma"):
+        kt[name] = c.kernel_time(name)
+    c.set_timing(False)
+    for _ in range(steps):
+        c.sync()
         t1 = time.perf_counter()
         ev = c.lowest_levels(k, LEVEL_TOL)
         c.sync()
-        t2 = time.perf_counter()
-        t_acc += t1 - t0
-        t_lev += t2 - t1
-    kt = {}
-    for name in ("sobol", "map", "pes", "weight", "mlp", "basis", "syrk", "syrk_mma", "symv", "symfill"):
-        kt[name] = c.kernel_time(name)
-    c.set_timing(False)
+        t_lev += time.perf_counter() - t1
     npts = c.points_done()
     return dict(t_acc=t_acc / steps, t_lev=t_lev / steps, levels=ev, kernels=kt, n=c.basis_size(),
                 points=npts, train_s=train_s)
