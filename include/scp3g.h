@@ -93,7 +93,7 @@ int scp3g_set_precision(scp3g_ctx *ctx, int mode);
  * selects the LDS-tiled FP64 kernel in the FP64 form. */
 int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast);
 /* points per device tile (multiple of 256), or 0 (default): the largest power
- * of two <= 524288 whose basis table fits in 8 GiB, at least 4096 */
+ * of two <= 524288 whose FP64 basis table fits in 8 GiB, at least 4096 */
 int scp3g_set_tile(scp3g_ctx *ctx, int tile);
 
 /* ---- whole-path entry points -------------------------------------- */

@@ -167,6 +167,7 @@ static void free_work(scp3g_ctx *ctx)
     dfree(ctx->d_B);
     ctx->work_tile = 0;
     ctx->work_n = 0;
+    ctx->work_precision = -1;
 }
 
 static void free_sums(scp3g_ctx *ctx)
@@ -178,9 +179,12 @@ static void free_sums(scp3g_ctx *ctx)
 
 // Points per tile when the caller sets none: per-point kernels need >= 64k
 // points to fill 256 CUs and the split-K SYRK partials amortise over the
-// tile, so take the largest power of two <= 2^19 whose FP64 table fits in
-// 8 GiB (HBM is 288 GB), but at least 4096.  (1225 states, 2^20 points:
-// 116, 124, 128, 128 M points/s at 2^17, 2^18, 2^19, 2^20.)
+// tile, so take the largest power of two <= 2^19 whose FP64 basis table
+// (rows_pad128 x T doubles) fits in 8 GiB (HBM is 288 GB), but at least
+// 4096.  Both forms use the same tile; the mixed form allocates only its
+// two BF16 planes.  (1225 states, 2^20 points: 116, 124, 128, 128 M
+// points/s at 2^17, 2^18, 2^19, 2^20; 20825 states: the mixed SYRK is 1.8%
+// slower per point at 65536 than at 32768.)
 static int64_t auto_tile(int64_t n)
 {
     const double rows = (double)rows_pad128(n) + 32;
@@ -193,7 +197,7 @@ static int ensure_work(scp3g_ctx *ctx)
 {
     const int64_t n = ctx->nbasis;
     const int64_t Tw = ctx->tile > 0 ? ctx->tile : auto_tile(n);
-    if (ctx->work_tile == Tw && ctx->work_n == n && ctx->d_B) return 0;
+    if (ctx->work_tile == Tw && ctx->work_n == n && ctx->work_precision == ctx->precision && ctx->d_B) return 0;
     free_work(ctx);
     const size_t T = (size_t)Tw;
     const size_t rows = (size_t)rows_pad128(n);  // FP64 table rows (zero padded)
@@ -204,12 +208,14 @@ static int ensure_work(scp3g_ctx *ctx)
         return rc;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_E, sizeof(double) * T), "alloc E"))) return rc;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_w, sizeof(double) * T), "alloc w"))) return rc;
-    // FP64 table (rows x T doubles) or, in the mixed form, the two BF16
-    // planes of X (rows padded to 256) and the sign words, whichever is larger
+    // the FP64 table (rows x T doubles) in the FP64 form; in the mixed form
+    // the two BF16 planes of X (rows padded to 256) and the sign words, which
+    // also hold the direct-load kernels' FP32 table (rows x T floats)
     const size_t rows_f = (size_t)fast_rows_pad(n);
-    const size_t bbytes = std::max(sizeof(double) * rows * T, 4 * rows_f * T + T / 8 + 256);
+    const size_t bbytes = ctx->precision == SCP3G_FP64 ? sizeof(double) * rows * T : 4 * rows_f * T + T / 8 + 256;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_B, bbytes), "alloc B"))) return rc;
     if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_B, 0, bbytes, ctx->stream), "clear B"))) return rc;
+    ctx->work_precision = ctx->precision;
     ctx->work_tile = (int)Tw;
     ctx->work_n = n;
     return 0;

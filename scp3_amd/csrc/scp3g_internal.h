@@ -77,6 +77,7 @@ struct scp3g_ctx {
 
     // per-tile work buffers (tile points)
     int work_tile = 0;
+    int work_precision = -1;   // precision the work buffers were sized for
     int64_t work_n = 0;
     double *d_y = nullptr;     // nmodes0 x tile
     void *d_crd = nullptr;     // 9nw x tile (double)
