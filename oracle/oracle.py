@@ -56,6 +56,12 @@ def lib():
         L.orc_qtip4pf.argtypes = [I64, P]
         L.orc_qtip4pf.restype = D
         L.orc_scp3_accumulate.argtypes = [I, I, P, P, P, I, P, I, I, I, I64, I64, P, P, P, P, P]
+        L.orc_grad_rule.argtypes = [I, P, P, P, P, D]
+        L.orc_grad_rule.restype = D
+        L.orc_grad_accumulate.argtypes = [I, I, I, I, I64, P, P, P, P]
+        L.orc_qtip4pf_grad.argtypes = [I64, P, P]
+        L.orc_qtip4pf_grad.restype = D
+        L.orc_scp3_accumulate_grad.argtypes = [I, I, P, P, P, I, P, I, I, I, I64, I64, P, P, P, P]
         if hasattr(L, "orc_ttm3f"):
             try:
                 L.orc_ttm3f.argtypes = [I64, P]
@@ -186,6 +192,55 @@ def scp3_accumulate(sysd, pes, seed0, npts, normals=None, want=("M",)):
     assert rc == 0, rc
     out.update(M=Mm, y=y, V=V, bV=bV)
     return out
+
+
+def qtip4pf_grad(crd):
+    """q-TIP4P/F energy (kcal/mol) and gradient (kcal/mol/A) of 9nw coordinates."""
+    c = np.ascontiguousarray(crd, dtype=np.float64).ravel()
+    g = np.zeros_like(c)
+    e = lib().orc_qtip4pf_grad(c.size // 9, _p(c), _p(g))
+    return e, g
+
+
+def grad_accumulate(q, V, vq, nsel, n2, n3, form=0):
+    """Raw gradient-estimator sums (n x n Fortran, upper) for points q, V, vq.
+
+    q, vq: points x nsel (selected modes); V: the weights bV.  form 0 is the
+    reference's element placement, 1 the Stein rule for every element.
+    """
+    q = np.ascontiguousarray(np.atleast_2d(q), dtype=np.float64)
+    vq = np.ascontiguousarray(np.atleast_2d(vq), dtype=np.float64)
+    V = np.ascontiguousarray(np.atleast_1d(V), dtype=np.float64)
+    n = basis_size(nsel, n2, n3)
+    M = np.zeros((n, n), order="F")
+    rc = lib().orc_grad_accumulate(nsel, n2, n3, form, q.shape[0], _p(q), _p(V), _p(vq), _p(M))
+    assert rc == 0, rc
+    return M
+
+
+def scp3_accumulate_grad(sysd, seed0, npts, form=0, normals=None):
+    """The gradient-estimator block loop (SCP3.cpp:187-232) with q-TIP4P/F.
+
+    Returns dict with the raw sums M (n x n Fortran, upper), bV and Vq
+    (points x selected modes).
+    """
+    L = lib()
+    nw = sysd["nw"]
+    M0 = sysd["omega"].size
+    modes = np.ascontiguousarray(sysd["modes"], dtype=np.int32)
+    n = basis_size(modes.size, sysd["nmodes2"], sysd["nmodes3"])
+    Mm = np.zeros((n, n), order="F")
+    bV = np.zeros(npts)
+    Vq = np.zeros((npts, modes.size))
+    x0 = np.ascontiguousarray(sysd["x0"], dtype=np.float64)
+    om = np.ascontiguousarray(sysd["omega"], dtype=np.float64)
+    MU = np.asfortranarray(sysd["MUa"], dtype=np.float64)
+    nr = np.ascontiguousarray(normals, dtype=np.float64) if normals is not None else None
+    rc = L.orc_scp3_accumulate_grad(nw, M0, _p(x0), _p(om), _p(MU), modes.size, _p(modes),
+                                    sysd["nmodes2"], sysd["nmodes3"], form, seed0, npts, _p(nr),
+                                    _p(Mm), _p(bV), _p(Vq))
+    assert rc == 0, rc
+    return dict(M=Mm, bV=bV, Vq=Vq)
 
 
 def finalize(sysd, Mraw, npts):
