@@ -46,6 +46,17 @@ typedef struct scp3g_ctx scp3g_ctx;
                        /* SYRK accumulated in FP32 per tile, folded into an   */
                        /* FP64 running H by compensated summation             */
 
+/* estimators of the potential-energy matrix M (SCP3.cpp:203-231) */
+#define SCP3G_EST_ENERGY 0          /* -G: M += B diag(bV) B^T (addEpot,       */
+                                    /* ScaledMatrixElements.cpp:951-971)       */
+#define SCP3G_EST_GRADIENT 1        /* the default without -G: the Stein-       */
+                                    /* identity elements from bV and its        */
+                                    /* gradient, placed as the reference does   */
+                                    /* (ScaledMatrixElements.cpp:100-725)       */
+#define SCP3G_EST_GRADIENT_STEIN 2  /* the same estimator with every element    */
+                                    /* from the Stein identity (differs from 1  */
+                                    /* in five element families, see grad.hip)  */
+
 /* potentials selectable on the GPU (h2o.h:20-48 names, plus ttm3f) */
 #define SCP3G_PES_QTIP4PF 0
 #define SCP3G_PES_TTM3F 1
@@ -92,6 +103,13 @@ int scp3g_set_precision(scp3g_ctx *ctx, int mode);
  * 0 = 64 x 64-tile direct-load kernels on B and w.  Any nonzero value
  * selects the LDS-tiled FP64 kernel in the FP64 form. */
 int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast);
+/*
+ * SCP3G_EST_ENERGY (default), SCP3G_EST_GRADIENT or SCP3G_EST_GRADIENT_STEIN
+ * (SCP3.cpp:63-151 option -G selects the first; use_gradient the others).
+ * Clears the running sum.  The gradient forms run in FP64 with the q-TIP4P/F
+ * potential and need nmodes3 <= nmodes2; accumulate calls fail otherwise.
+ */
+int scp3g_set_estimator(scp3g_ctx *ctx, int estimator);
 /* points per device tile (multiple of 256), or 0 (default): the largest power
  * of two <= 524288 whose FP64 basis table fits in 8 GiB, at least 4096 */
 int scp3g_set_tile(scp3g_ctx *ctx, int tile);
@@ -229,6 +247,14 @@ int scp3g_map_coords(scp3g_ctx *ctx, const double *y_dev, int64_t ldy, int64_t n
  */
 int scp3g_pes_energy(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const void *crd_dev,
                      int64_t ldc, int fp32, double *E_dev);
+
+/*
+ * Energies and gradients of npts configurations (FP64): E_dev[i] as above and
+ * G_dev[c*ldg + i] = dE/dx_c in kcal/mol/A, the gradient operator()
+ * (PES.h:13-23; qtip4pf.cpp:252-397).  Only SCP3G_PES_QTIP4PF.
+ */
+int scp3g_pes_gradient(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const double *crd_dev,
+                       int64_t ldc, double *E_dev, double *G_dev, int64_t ldg);
 
 /*
  * Basis table B_dev[a*ldb + i] = bra_a(q_i) for the context's basis, with

@@ -17,6 +17,9 @@ FP64 = 0
 MIXED = 1
 PES_QTIP4PF = 0
 PES_TTM3F = 1
+EST_ENERGY = 0          # -G: B^T diag(bV) B
+EST_GRADIENT = 1        # the reference's default estimator (bV and its gradient)
+EST_GRADIENT_STEIN = 2  # the same with every element from the Stein identity
 
 # Constants.cpp:12-18, Constants.h:21
 BOHR = 0.52917721092
@@ -44,6 +47,10 @@ _SIGS = {
     "scp3g_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_mixed_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_cv_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "scp3g_set_estimator": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "scp3g_pes_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                          ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_int64]),
     "scp3g_accumulate_surrogate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "scp3g_points_coords": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
     "scp3g_accumulate_energies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
@@ -189,6 +196,10 @@ class Context:
 
     def set_tile(self, tile):
         self.check(self.L.scp3g_set_tile(self._h, tile))
+
+    def set_estimator(self, est):
+        """EST_ENERGY (-G), EST_GRADIENT or EST_GRADIENT_STEIN; clears the running sum."""
+        self.check(self.L.scp3g_set_estimator(self._h, int(est)))
 
     def set_mixed_kernel(self, kernel):
         """0: 64 x 64 direct-load SYRKs; 1, 2: LDS-tiled SYRKs (see scp3g.h)"""

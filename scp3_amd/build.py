@@ -26,6 +26,7 @@ SOURCES = [
     "syrk64_fast.hip",
     "eigen.hip",
     "mlp.hip",
+    "grad.hip",
     "context.cpp",
     "modes.cpp",
     "pes_cpu.cpp",

@@ -165,15 +165,22 @@ static void free_work(scp3g_ctx *ctx)
     dfree(ctx->d_E);
     dfree(ctx->d_w);
     dfree(ctx->d_B);
+    dfree(ctx->d_G);
+    dfree(ctx->d_Vq);
+    dfree(ctx->d_wp);
+    dfree(ctx->d_wm);
+    dfree(ctx->d_sc);
     ctx->work_tile = 0;
     ctx->work_n = 0;
     ctx->work_precision = -1;
+    ctx->work_estimator = -1;
 }
 
 static void free_sums(scp3g_ctx *ctx)
 {
     dfree(ctx->d_H);
     dfree(ctx->d_Hc);
+    dfree(ctx->d_F);
     ctx->h_n = 0;
 }
 
@@ -197,7 +204,10 @@ static int ensure_work(scp3g_ctx *ctx)
 {
     const int64_t n = ctx->nbasis;
     const int64_t Tw = ctx->tile > 0 ? ctx->tile : auto_tile(n);
-    if (ctx->work_tile == Tw && ctx->work_n == n && ctx->work_precision == ctx->precision && ctx->d_B) return 0;
+    const int grad = ctx->estimator != SCP3G_EST_ENERGY;
+    if (ctx->work_tile == Tw && ctx->work_n == n && ctx->work_precision == ctx->precision &&
+        (ctx->work_estimator != SCP3G_EST_ENERGY) == grad && ctx->work_estimator >= 0 && ctx->d_B)
+        return 0;
     free_work(ctx);
     const size_t T = (size_t)Tw;
     const size_t rows = (size_t)rows_pad128(n);  // FP64 table rows (zero padded)
@@ -215,6 +225,17 @@ static int ensure_work(scp3g_ctx *ctx)
     const size_t bbytes = ctx->precision == SCP3G_FP64 ? sizeof(double) * rows * T : 4 * rows_f * T + T / 8 + 256;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_B, bbytes), "alloc B"))) return rc;
     if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_B, 0, bbytes, ctx->stream), "clear B"))) return rc;
+    if (grad) {
+        // gradient estimator: dE/dx, Vq over the selected modes, weights, scale
+        const size_t nsel = ctx->modes.size();
+        if ((rc = check_hip(ctx, hipMalloc(&ctx->d_G, sizeof(double) * 9 * ctx->nw * T), "alloc G")))
+            return rc;
+        if ((rc = check_hip(ctx, hipMalloc(&ctx->d_Vq, sizeof(double) * nsel * T), "alloc Vq"))) return rc;
+        if ((rc = check_hip(ctx, hipMalloc(&ctx->d_wp, sizeof(double) * T), "alloc wp"))) return rc;
+        if ((rc = check_hip(ctx, hipMalloc(&ctx->d_wm, sizeof(double) * T), "alloc wm"))) return rc;
+        if ((rc = check_hip(ctx, hipMalloc(&ctx->d_sc, sizeof(double) * T), "alloc sc"))) return rc;
+    }
+    ctx->work_estimator = ctx->estimator;
     ctx->work_precision = ctx->precision;
     ctx->work_tile = (int)Tw;
     ctx->work_n = n;
@@ -224,15 +245,23 @@ static int ensure_work(scp3g_ctx *ctx)
 static int ensure_sums(scp3g_ctx *ctx)
 {
     const int64_t n = ctx->nbasis;
-    if (ctx->h_n == n && ctx->d_H && ctx->d_Hc) return 0;
+    const bool grad = ctx->estimator != SCP3G_EST_ENERGY;
+    int rc;
+    if (grad && (rc = grad_setup(ctx))) return rc;
+    if (ctx->h_n == n && ctx->d_H && ctx->d_Hc && (!grad || ctx->d_F)) return 0;
     free_sums(ctx);
     const size_t bytes = sizeof(double) * (size_t)n * (size_t)n;
-    int rc;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_H, bytes), "alloc H"))) return rc;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_Hc, bytes), "alloc Hc"))) return rc;
     if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_H, 0, bytes, ctx->stream), "clear H"))) return rc;
     if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_Hc, 0, bytes, ctx->stream), "clear Hc")))
         return rc;
+    if (grad) {
+        if ((rc = check_hip(ctx, hipMalloc(&ctx->d_F, sizeof(double) * ctx->ngfix), "alloc F"))) return rc;
+        if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_F, 0, sizeof(double) * ctx->ngfix, ctx->stream),
+                            "clear F")))
+            return rc;
+    }
     ctx->h_n = n;
     ctx->points_done = 0;
     ctx->norm_sum = 0.0;
@@ -283,6 +312,7 @@ void scp3g_destroy(scp3g_ctx *ctx)
     for (hipEvent_t ev : ctx->event_pool) hipEventDestroy(ev);
     free_work(ctx);
     free_sums(ctx);
+    grad_release(ctx);
     dfree(ctx->d_part);
     dfree(ctx->d_sobol);
     dfree(ctx->d_x0);
@@ -383,6 +413,7 @@ int scp3g_set_basis(scp3g_ctx *ctx, int nsel, const int *modes, int nmodes2, int
     if (rc) return rc;
     free_work(ctx);
     free_sums(ctx);
+    grad_release(ctx);
     return 0;
 }
 
@@ -445,6 +476,21 @@ int scp3g_set_cv_terms(scp3g_ctx *ctx, int terms)
     return 0;
 }
 
+int scp3g_set_estimator(scp3g_ctx *ctx, int estimator)
+{
+    if (!ctx) return -1;
+    if (estimator != SCP3G_EST_ENERGY && estimator != SCP3G_EST_GRADIENT &&
+        estimator != SCP3G_EST_GRADIENT_STEIN)
+        return fail(ctx, -4, "bad estimator");
+    if (estimator != ctx->estimator) {
+        hipSetDevice(ctx->device);
+        ctx->estimator = estimator;
+        free_sums(ctx);
+        grad_release(ctx);
+    }
+    return 0;
+}
+
 int scp3g_set_tile(scp3g_ctx *ctx, int tile)
 {
     if (!ctx) return -1;
@@ -464,6 +510,9 @@ int scp3g_reset(scp3g_ctx *ctx)
     if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_H, 0, bytes, ctx->stream), "clear H"))) return rc;
     if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_Hc, 0, bytes, ctx->stream), "clear Hc")))
         return rc;
+    if (ctx->d_F &&
+        (rc = check_hip(ctx, hipMemsetAsync(ctx->d_F, 0, sizeof(double) * ctx->ngfix, ctx->stream), "clear F")))
+        return rc;
     ctx->points_done = 0;
     ctx->norm_sum = 0.0;
     return 0;
@@ -476,12 +525,85 @@ int scp3g_reset(scp3g_ctx *ctx)
 // `scale`; the normaliser grows by np for modes 0 and 3, np*scale for 1.
 }  // extern "C"
 
+// The gradient estimator's tile (SCP3.cpp:205-231 with use_gradient):
+// map, q-TIP4P/F energy and gradient, bV, Vq, the two FP64 SYRKs of
+// phi +- s Psi, and the block corrections and fixed elements.
+static int run_tile_grad(scp3g_ctx *ctx, int64_t np)
+{
+    const int64_t T = ctx->work_tile;
+    hipEvent_t ev;
+    int rc;
+    timer_begin(ctx, "map", &ev);
+    rc = check_hip(ctx, launch_map(ctx, ctx->d_y, T, np, ctx->d_crd, T, 0), "map");
+    timer_end(ctx, "map", ev);
+    if (rc) return rc;
+    timer_begin(ctx, "pes", &ev);
+    rc = check_hip(ctx, launch_qtip4pf_grad(ctx, ctx->nw, np, (const double *)ctx->d_crd, T, ctx->d_E,
+                                            ctx->d_G, T),
+                   "pes gradient");
+    timer_end(ctx, "pes", ev);
+    if (rc) return rc;
+    timer_begin(ctx, "weight", &ev);
+    rc = check_hip(ctx, launch_weight(ctx, ctx->d_E, ctx->d_y, T, np, ctx->d_w), "weight");
+    if (!rc)
+        rc = check_hip(ctx, launch_vq(ctx, ctx->d_G, T, ctx->d_y, T, ctx->d_w, np, ctx->d_Vq, T, ctx->d_wp,
+                                      ctx->d_wm, ctx->d_sc),
+                       "vq");
+    timer_end(ctx, "weight", ev);
+    if (rc) return rc;
+    double *B = (double *)ctx->d_B;
+    for (int pass = 0; pass < 2; ++pass) {
+        timer_begin(ctx, "basis", &ev);
+        rc = check_hip(ctx, launch_grad_table(ctx, ctx->d_y, T, np, ctx->d_w, ctx->d_Vq, T, ctx->d_sc,
+                                              pass ? -1.0 : 1.0, B, T),
+                       "grad table");
+        timer_end(ctx, "basis", ev);
+        if (rc) return rc;
+        timer_begin(ctx, "syrk", &ev);
+        rc = check_hip(ctx, launch_syrk_f64_lds(ctx, ctx->nbasis, B, T, pass ? ctx->d_wm : ctx->d_wp, np,
+                                                ctx->d_H),
+                       "syrk");
+        timer_end(ctx, "syrk", ev);
+        if (rc) return rc;
+    }
+    timer_begin(ctx, "grad_blocks", &ev);
+    rc = check_hip(ctx, launch_grad_corrections(ctx, ctx->d_y, T, np, ctx->d_w, ctx->d_Vq, T, ctx->d_H,
+                                                ctx->d_F),
+                   "grad corrections");
+    timer_end(ctx, "grad_blocks", ev);
+    if (rc) return rc;
+    ctx->points_done += np;
+    ctx->norm_sum += (double)np;
+    return 0;
+}
+
+int scp3g::read_sums(scp3g_ctx *ctx, const double **H)
+{
+    if (ctx->estimator == SCP3G_EST_ENERGY) {
+        *H = ctx->d_H;
+        return 0;
+    }
+    int rc = ensure_sums(ctx);
+    if (rc) return rc;
+    rc = check_hip(ctx, launch_grad_effective(ctx, ctx->d_H, ctx->d_F, ctx->d_Hc), "grad effective");
+    *H = ctx->d_Hc;
+    return rc;
+}
+
 int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
 {
     const int64_t T = ctx->work_tile;
     const int fp32 = ctx->precision == SCP3G_MIXED;
     hipEvent_t ev;
     int rc;
+    if (ctx->estimator != SCP3G_EST_ENERGY) {
+        if (wmode != 0) return fail(ctx, -4, "the gradient estimator has no surrogate or energy-input form");
+        if (ctx->precision != SCP3G_FP64) return fail(ctx, -4, "the gradient estimator runs in FP64");
+        if (ctx->pes != SCP3G_PES_QTIP4PF)
+            return fail(ctx, -4, "the gradient estimator needs potential gradients on the GPU: qtip4pf only");
+        if (ctx->nmodes3 > ctx->nmodes2) return fail(ctx, -4, "the gradient estimator needs nmodes3 <= nmodes2");
+        return run_tile_grad(ctx, np);
+    }
     if (wmode == 1) {
         timer_begin(ctx, "mlp", &ev);
         rc = check_hip(ctx, launch_mlp_weights(ctx, ctx->d_y, T, np, nullptr, 0, scale, ctx->d_w), "mlp");
@@ -676,9 +798,11 @@ int scp3g_h_download(scp3g_ctx *ctx, double *M_host, int raw)
     if ((rc = ensure_sums(ctx))) return rc;
     const int64_t n = ctx->nbasis;
     const size_t bytes = sizeof(double) * (size_t)n * (size_t)n;
+    const double *Hs = nullptr;
+    if ((rc = read_sums(ctx, &Hs))) return rc;
     if (raw) {
         if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync"))) return rc;
-        return check_hip(ctx, hipMemcpy(M_host, ctx->d_H, bytes, hipMemcpyDeviceToHost), "download H");
+        return check_hip(ctx, hipMemcpy(M_host, Hs, bytes, hipMemcpyDeviceToHost), "download H");
     }
     if (ctx->points_done <= 0) return fail(ctx, -6, "no points accumulated");
     std::vector<double> dg = ho_diagonal(ctx);
@@ -691,7 +815,7 @@ int scp3g_h_download(scp3g_ctx *ctx, double *M_host, int raw)
     hipMemcpy(d_diag, dg.data(), n * 8, hipMemcpyHostToDevice);
     hipEvent_t ev;
     timer_begin(ctx, "finalize", &ev);
-    rc = check_hip(ctx, launch_finalize(ctx, ctx->d_H, n, ctx->norm_sum, d_diag, d_out),
+    rc = check_hip(ctx, launch_finalize(ctx, Hs, n, ctx->norm_sum, d_diag, d_out),
                    "finalize");
     timer_end(ctx, "finalize", ev);
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
@@ -758,6 +882,21 @@ int scp3g_pes_energy(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const void *
         rc = check_hip(ctx, launch_ttm3f(ctx, nw, npts, crd_dev, ldc, fp32, E_dev), "ttm3f");
     else
         rc = fail(ctx, -4, "unknown potential id");
+    timer_end(ctx, "pes", ev);
+    return rc;
+}
+
+int scp3g_pes_gradient(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const double *crd_dev,
+                       int64_t ldc, double *E_dev, double *G_dev, int64_t ldg)
+{
+    if (!ctx) return -1;
+    if (nw <= 0 || nw > 64 || npts < 0 || ldc < npts || ldg < npts || !crd_dev || !E_dev || !G_dev)
+        return fail(ctx, -4, "bad arguments");
+    if (pes != SCP3G_PES_QTIP4PF) return fail(ctx, -4, "gradients on the GPU: qtip4pf only");
+    hipSetDevice(ctx->device);
+    hipEvent_t ev;
+    timer_begin(ctx, "pes", &ev);
+    int rc = check_hip(ctx, launch_qtip4pf_grad(ctx, nw, npts, crd_dev, ldc, E_dev, G_dev, ldg), "qtip4pf");
     timer_end(ctx, "pes", ev);
     return rc;
 }

@@ -840,7 +840,10 @@ extern "C" int scp3g_lowest_levels(scp3g_ctx *ctx, int k, double tol, double *ev
     if (!ctx) return -1;
     if (ctx->nbasis <= 0 || !ctx->d_H) return fail(ctx, -2, "no Hamiltonian accumulated");
     if (ctx->points_done <= 0) return fail(ctx, -6, "no points accumulated");
-    return davidson_run(ctx, ctx->d_H, ctx->nbasis, ctx->nbasis, ctx->norm_sum,
+    const double *Hs = nullptr;
+    int rc = read_sums(ctx, &Hs);
+    if (rc) return rc;
+    return davidson_run(ctx, Hs, ctx->nbasis, ctx->nbasis, ctx->norm_sum,
                         ho_diagonal(ctx), k, tol, evals, evecs);
 }
 
@@ -853,7 +856,10 @@ extern "C" int scp3g_lowest_levels_sub(scp3g_ctx *ctx, int64_t nsub, int k, doub
     if (nsub < 1 || nsub > ctx->nbasis) return fail(ctx, -4, "nsub out of range");
     std::vector<double> dg = ho_diagonal(ctx);
     dg.resize((size_t)nsub);
-    return davidson_run(ctx, ctx->d_H, ctx->nbasis, nsub, ctx->norm_sum, dg, k, tol,
+    const double *Hs = nullptr;
+    int rc = read_sums(ctx, &Hs);
+    if (rc) return rc;
+    return davidson_run(ctx, Hs, ctx->nbasis, nsub, ctx->norm_sum, dg, k, tol,
                         evals, nullptr);
 }
 

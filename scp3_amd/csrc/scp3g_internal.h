@@ -68,6 +68,8 @@ struct scp3g_ctx {
     int x3_variant = 1;
     // control variate, mixed form: the surrogate term with hi.hi products only
     bool cv_one_term = false;
+    // SCP3G_EST_ENERGY (the -G estimator) or one of the gradient forms
+    int estimator = SCP3G_EST_ENERGY;
 
     // device copies of the set-up
     uint64_t *d_sobol = nullptr;  // nmodes0 x 62
@@ -84,12 +86,25 @@ struct scp3g_ctx {
     double *d_E = nullptr;     // tile
     double *d_w = nullptr;     // tile
     void *d_B = nullptr;       // n x tile (double or float)
+    // gradient estimator, per tile: dE/dx (9nw x tile), Vq (nsel x tile),
+    // the +-SYRK weights and the per-point scale
+    int work_estimator = -1;
+    double *d_G = nullptr, *d_Vq = nullptr, *d_wp = nullptr, *d_wm = nullptr, *d_sc = nullptr;
 
     // running sums
     double *d_H = nullptr;     // n x n column-major, upper
     double *d_Hc = nullptr;    // compensation (mixed mode)
     double *d_part = nullptr;  // split-K partials
     size_t part_bytes = 0;
+    // gradient estimator: running sums of the fixed elements (ngfix), the
+    // A_d row lists, correction work items and fix list (grad.hip)
+    double *d_F = nullptr;
+    int *d_grows = nullptr;
+    int4 *d_gwork = nullptr, *d_gfix = nullptr;
+    double *d_gpart = nullptr;
+    size_t gpart_bytes = 0;
+    int ngwork = 0, ngfix = 0, grad_form = -1;
+    int64_t grad_n = 0;
     int64_t h_n = 0;
     int64_t points_done = 0;
     // normaliser of the running sum: the number of points for the plain
@@ -159,6 +174,24 @@ hipError_t launch_syrk_f64_lds(scp3g_ctx *ctx, int64_t n, const double *B, int64
                                const double *w, int64_t npts, double *H);
 void release_tiles128(scp3g_ctx *ctx);
 double qtip4pf_energy_host(int nw, const double *crd);
+hipError_t launch_qtip4pf_grad(scp3g_ctx *ctx, int nw, int64_t npts, const double *crd, int64_t ldc,
+                               double *E, double *G, int64_t ldg);
+// gradient estimator (grad.hip)
+int grad_setup(scp3g_ctx *ctx);
+void grad_release(scp3g_ctx *ctx);
+hipError_t launch_vq(scp3g_ctx *ctx, const double *G, int64_t ldg, const double *y, int64_t ldy,
+                     const double *w, int64_t npts, double *Vq, int64_t ldv, double *wp, double *wm,
+                     double *sc);
+hipError_t launch_grad_table(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts,
+                             const double *w, const double *Vq, int64_t ldv, const double *sc,
+                             double sgn, double *B, int64_t ldb);
+hipError_t launch_grad_corrections(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts,
+                                   const double *w, const double *Vq, int64_t ldv, double *H,
+                                   double *F);
+hipError_t launch_grad_effective(scp3g_ctx *ctx, const double *S, const double *F, double *dst);
+// the running sum as the reference's M (before division by the points):
+// ctx->d_H for the energy estimator, built into ctx->d_Hc for the gradient one
+int read_sums(scp3g_ctx *ctx, const double **H);
 double ttm3f_energy_host(int nw, const double *crd, int *iters);
 void mlp_release(scp3g_ctx *ctx);
 void release_davidson(scp3g_ctx *ctx);

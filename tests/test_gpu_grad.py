@@ -1,0 +1,137 @@
+"""Gradient (Stein-identity) estimator on the GPU vs the CPU oracle, through the C ABI.
+
+Bars: q-TIP4P/F energies to 1e-12 relative and gradients to 1e-10 of their
+largest component against oracle/grad_oracle.c; the raw running sum M
+(before division by the points) within 1e-9 of max|M| of the oracle's on
+the same Sobol points, for the reference's element placement and for the
+Stein form; the lowest levels within 1e-8 hartree of LAPACK on the oracle's
+finalised matrix.  The FP64 sums differ from the oracle's only in summation
+order (split-K partials, the +-(phi +- s Psi) SYRK pair and the block
+corrections).
+"""
+import numpy as np
+import pytest
+
+import scp3_amd
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+# oracle form numbers: 0 = the reference's placement, 1 = the Stein rule
+FORMS = {scp3_amd.EST_GRADIENT: 0, scp3_amd.EST_GRADIENT_STEIN: 1}
+
+
+def vp(t):
+    import ctypes
+    torch.cuda.synchronize()
+    return ctypes.c_void_p(t.data_ptr())
+
+
+@pytest.fixture(scope="module")
+def ctx(gpu_ctx_factory):
+    return gpu_ctx_factory(0)
+
+
+def setup(c, sysd, est, tile=4096):
+    c.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
+    c.set_basis(sysd["modes"], sysd["nmodes2"], sysd["nmodes3"])
+    c.set_pes("qtip4pf")
+    c.set_precision(scp3_amd.FP64)
+    c.set_tile(tile)
+    c.set_estimator(est)
+
+
+@pytest.mark.parametrize("which", ["dimer", "hexamer"])
+def test_qtip4pf_gradient_matches_oracle(ctx, which, dimer, hexamer):
+    sysd = dimer if which == "dimer" else hexamer
+    print("qtip4pf gradient", which, flush=True)
+    rng = np.random.default_rng(2)
+    n3 = sysd["x0"].size
+    nw = n3 // 9
+    P = 1000
+    X = sysd["x0"][None, :] * scp3_amd.BOHR + rng.normal(size=(P, n3)) * 0.04
+    ld = 1024
+    crd = torch.zeros(n3 * ld, dtype=torch.float64, device="cuda")
+    crd.view(n3, ld)[:, :P] = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    E = torch.zeros(ld, dtype=torch.float64, device="cuda")
+    G = torch.zeros(n3 * ld, dtype=torch.float64, device="cuda")
+    ctx.check(ctx.L.scp3g_pes_gradient(ctx.handle, scp3_amd.PES_QTIP4PF, nw, P, vp(crd), ld, vp(E), vp(G), ld))
+    ctx.sync()
+    Eg = E.cpu().numpy()[:P]
+    Gg = G.cpu().numpy().reshape(n3, ld)[:, :P].T
+    for i in range(0, P, 37):
+        e, g = oracle.qtip4pf_grad(X[i])
+        assert Eg[i] == pytest.approx(e, rel=1e-12, abs=1e-11)
+        np.testing.assert_allclose(Gg[i], g, rtol=0, atol=1e-10 * np.max(np.abs(g)))
+
+
+def _check_sums(ctx, sysd, est, seed, npts, tile=4096):
+    setup(ctx, sysd, est, tile)
+    ctx.accumulate(seed, npts)
+    M = ctx.h_download(raw=True)
+    ref = oracle.scp3_accumulate_grad(sysd, seed, npts, form=FORMS[est])["M"]
+    U = np.triu(np.ones(M.shape, bool))
+    scale = np.max(np.abs(ref[U]))
+    err = np.max(np.abs((M - ref)[U]))
+    print("grad sums", est, M.shape[0], npts, "max err / max", err / scale, flush=True)
+    assert err <= 1e-9 * scale
+    assert np.all(M[~U] == 0.0)
+    return M, ref
+
+
+@pytest.mark.parametrize("est", [scp3_amd.EST_GRADIENT, scp3_amd.EST_GRADIENT_STEIN])
+def test_grad_sums_dimer(ctx, dimer, est):
+    # 5000 points: one full 4096-point tile and a ragged second one
+    _check_sums(ctx, dimer, est, 1 << 30, 5000)
+
+
+@pytest.mark.parametrize("est", [scp3_amd.EST_GRADIENT, scp3_amd.EST_GRADIENT_STEIN])
+def test_grad_sums_dimer_triples(ctx, est):
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path("dimer_qtip4pf"), nmodes2=12, nmodes3=7)
+    _check_sums(ctx, sysd, est, 12345, 3000)
+
+
+def test_grad_sums_hexamer(ctx, hexamer):
+    _check_sums(ctx, hexamer, scp3_amd.EST_GRADIENT, 1 << 30, 300)
+
+
+def test_grad_levels_dimer(ctx, dimer):
+    import scipy.linalg
+    npts = 4096
+    setup(ctx, dimer, scp3_amd.EST_GRADIENT)
+    ctx.accumulate(1 << 30, npts)
+    ev = ctx.lowest_levels(5, tol=1e-10)
+    Mout = ctx.h_download()
+    ref = oracle.finalize(dimer, oracle.scp3_accumulate_grad(dimer, 1 << 30, npts, form=0)["M"], npts)
+    np.testing.assert_allclose(np.triu(Mout), np.triu(ref), rtol=0, atol=1e-12 * np.max(np.abs(ref)))
+    w = scipy.linalg.eigh(ref, lower=False, eigvals_only=True, subset_by_index=[0, 4])
+    np.testing.assert_allclose(ev, w, rtol=0, atol=1e-8)
+
+
+def test_grad_normals_branch(ctx, dimer):
+    """The -r branch (caller's standard normals) through the gradient estimator."""
+    rng = np.random.default_rng(9)
+    npts = 700
+    Z = rng.normal(size=(npts, dimer["omega"].size))
+    setup(ctx, dimer, scp3_amd.EST_GRADIENT)
+    ctx.accumulate_normals(Z)
+    M = ctx.h_download(raw=True)
+    ref = oracle.scp3_accumulate_grad(dimer, 0, npts, form=0, normals=Z)["M"]
+    U = np.triu(np.ones(M.shape, bool))
+    assert np.max(np.abs((M - ref)[U])) <= 1e-9 * np.max(np.abs(ref[U]))
+
+
+def test_grad_estimator_rejects_unsupported(ctx, dimer):
+    setup(ctx, dimer, scp3_amd.EST_GRADIENT)
+    ctx.set_precision(scp3_amd.MIXED)
+    with pytest.raises(scp3_amd.Scp3Error, match="FP64"):
+        ctx.accumulate(0, 256)
+    ctx.set_precision(scp3_amd.FP64)
+    ctx.set_pes("ttm3f")
+    with pytest.raises(scp3_amd.Scp3Error, match="qtip4pf"):
+        ctx.accumulate(0, 256)
+    ctx.set_pes("qtip4pf")
+    ctx.set_estimator(scp3_amd.EST_ENERGY)
