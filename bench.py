@@ -21,6 +21,11 @@ control variate: 2^24 surrogate + 2^19 true points against plain FP64 at
 174-512-512-512-1 on 2^19 pairs, 2^24 surrogate + 2^20 true points against
 plain FP64 at 2^20, lowest 20 levels).
 
+Forms: fp64, mixed, cv (above) and grad / stein: the gradient (Stein-
+identity) estimator, the reference's default without -G, in FP64 with the
+q-TIP4P/F potential (grad: the reference's element placement; stein: every
+element from the Stein identity), e.g. --preset c2 --forms grad.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--preset c2]
 """
 import argparse
@@ -85,8 +90,13 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": n * 4, "check": float(dst[-1].item())}))
 
 
-def setup(c, sysd, pr, mode):
+GRAD_FORMS = ("grad", "stein")
+
+
+def setup(c, sysd, pr, mode, form=None):
     import scp3_amd
+    c.set_estimator({"grad": scp3_amd.EST_GRADIENT, "stein": scp3_amd.EST_GRADIENT_STEIN}.get(
+        form, scp3_amd.EST_ENERGY))
     c.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
     c.set_basis(sysd["modes"], sysd["nmodes2"], sysd["nmodes3"])
     c.set_pes("ttm3f" if "ttm3f" in pr["config"] else "qtip4pf")
@@ -98,8 +108,8 @@ def setup(c, sysd, pr, mode):
 
 def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
     import scp3_amd
-    mode = scp3_amd.FP64 if form == "fp64" else scp3_amd.MIXED
-    setup(c, sysd, pr, mode)
+    mode = scp3_amd.FP64 if form in ("fp64",) + GRAD_FORMS else scp3_amd.MIXED
+    setup(c, sysd, pr, mode, form)
     train_s = 0.0
     if form == "cv":
         t0 = time.perf_counter()
@@ -137,7 +147,7 @@ def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
         c.sync()
         t_acc += time.perf_counter() - t0
     kt = {}
-    for name in ("sobol", "map", "pes", "weight", "mlp", "basis", "syrk", "syrk_mma"):
+    for name in ("sobol", "map", "pes", "weight", "mlp", "basis", "syrk", "syrk_mma", "grad_blocks"):
         kt[name] = c.kernel_time(name)
     c.set_timing(False)
     for _ in range(steps):
@@ -248,9 +258,11 @@ def main():
     # apart); algorithmic flops of a symmetric rank-K update: n(n+1) per point
     ms, cnt = r["kernels"]["syrk_mma"]
     avg_s = ms / 1e3 / max(cnt, 1)
-    pts_per_launch = r["points"] * a.steps / max(cnt, 1)  # points_done counts one step
+    # points_done counts one step; the gradient forms run two SYRKs per tile
+    pts_per_launch = r["points"] * a.steps / max(cnt, 1) * (2 if main_form in GRAD_FORMS else 1)
     flops = float(n) * (n + 1) * pts_per_launch
-    peak = PEAK_FP64_TFLOPS if main_form == "fp64" else PEAK_BF16_TFLOPS
+    fp64_main = main_form in ("fp64",) + GRAD_FORMS
+    peak = PEAK_FP64_TFLOPS if fp64_main else PEAK_BF16_TFLOPS
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
     out = {
         "metric": "Sobol pts/s into H + time to lowest-10 levels at 0.5 cm^-1, GPU forms vs plain",
@@ -263,7 +275,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 H; bf16x3 MFMA SYRK, f32 potential/basis" if main_form != "fp64" else "f64",
+        "dtype": "f64" if fp64_main else "f64 H; bf16x3 MFMA SYRK, f32 potential/basis",
         "data": "synthetic: seed-built relaxed cluster (scp3_amd/data), Sobol points",
         "config": {"workload": "%s: %s, %d states, 2^%d Sobol points, %d-point tiles, lowest %d levels"
                                % (a.preset, pr["config"], n, pr["points"], round(pts_per_launch), pr["levels"]),
@@ -272,7 +284,7 @@ def main():
                      "frac": achieved / peak,
                      "traffic": pmc_traffic("k_syrk_x3<3>", pr["config"], n, pts_per_launch)
                      if main_form == "mixed" else None,
-                     "kernel": "%s (%s form)" % ("k_syrk_f64_lds" if main_form == "fp64" else "k_syrk_x3", main_form),
+                     "kernel": "%s (%s form)" % ("k_syrk_f64_lds" if fp64_main else "k_syrk_x3", main_form),
                      "avg_launch_ms": avg_s * 1e3,
                      "flops_per_launch": flops},
         "forms": {},

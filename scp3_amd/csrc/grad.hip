@@ -24,7 +24,8 @@
 //     (phi + s Psi) with weight 1/(2s) and (phi - s Psi) with weight -1/(2s),
 //     s per point = 1/max(|V|, |V_d|) so both terms are O(phi);
 //   * the block corrections: for each mode d, the |A_d| x |A_d| upper
-//     triangle over the basis functions A_d that excite d (k_grad_blocks);
+//     triangle over the basis functions A_d that excite d (k_grad_blocks,
+//     FP64 VALU on 64 x 64 tiles);
 //   * M(0,0) = sum V (ScaledMatrixElements.cpp:717) and the reference's own
 //     element placements (below) as a short list of per-point sums
 //     (k_grad_fix);
@@ -48,7 +49,7 @@ namespace scp3g {
 constexpr double kR2 = 1.4142135623730951;  // sqrt(2)
 constexpr double kR3 = 1.7320508075688772;  // sqrt(3)
 constexpr double kR6 = 2.4494897427831781;  // sqrt(6)
-constexpr int kGT = 32;                     // correction tile (rows of A_d)
+constexpr int kGT = 64;                     // correction tile (rows of A_d)
 constexpr int kGP = 16;                     // points per staged batch
 constexpr int kFixPerThread = 24;           // per-point sums per thread of k_grad_fix
 
@@ -173,11 +174,14 @@ __global__ void __launch_bounds__(256)
 
 // ---- block corrections --------------------------------------------------------
 //
-// Work item (d, ti, tj): rows ti*32.. and columns tj*32.. (ti <= tj) of the
-// upper triangle of A_d; blockIdx.y takes a chunk of kchunk points.  Per
-// batch of 16 points the block stages q (all selected modes), then u^d and
-// phi of its 64 rows, then each thread adds w_d u u - V phi phi to its four
-// elements.  Partial sums go to part[split][work][32 x 32].
+// Work item (d, ti, tj): rows ti*64.. and columns tj*64.. (ti <= tj) of the
+// upper triangle of A_d; blockIdx.y takes a chunk of kchunk points.  Thread
+// (tr, tc) = (tid / 16, tid % 16) owns the 4 x 4 elements (tr + 16i,
+// tc + 16j).  Per batch of 16 points the block stages q (all selected
+// modes) and w_d, V; each thread then evaluates u^d and phi of the eight
+// rows tr + 16j (I rows for j < 4, J rows after) at point tc, storing
+// w_d u and V phi for the I side; the 16 points are then contracted from
+// LDS.  Partial sums go to part[split][work][64 x 64].
 __global__ void __launch_bounds__(256)
     k_grad_blocks(const double *__restrict__ y, int64_t ldy, const int *__restrict__ modes, int nsel,
                   const RowDesc *__restrict__ desc, const int *__restrict__ rows,
@@ -186,16 +190,38 @@ __global__ void __launch_bounds__(256)
                   double *__restrict__ part)
 {
     extern __shared__ double qs[];  // [nsel][kGP]
-    __shared__ double uI[kGT][kGP + 1], fI[kGT][kGP + 1], uJ[kGT][kGP + 1], fJ[kGT][kGP + 1];
+    // [point][row], rows padded by one double so that the staging stores of
+    // 16 points x 4 rows per wave fall in distinct banks
+    __shared__ double aI[kGP][kGT + 1], bI[kGP][kGT + 1], uJ[kGP][kGT + 1], fJ[kGP][kGT + 1];
     __shared__ double wd[kGP], vv[kGP];
     const int4 wk = work[blockIdx.x];  // {offset of A_d, |A_d|, d, ti | tj << 16}
     const int off = wk.x, na = wk.y, d = wk.z, ti = wk.w & 0xffff, tj = wk.w >> 16;
-    const int tid = threadIdx.x;
-    const int r = tid >> 3, c0 = (tid & 7) * 4;
+    const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+    // this thread's staging rows: tr + 16j, I side for j < 4
+    uint32_t f0[8], f1[8], f2[8];
+    int slot[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int e = ((j < 4) ? ti : tj) * kGT + tr + 16 * (j & 3);
+        slot[j] = -1;
+        f0[j] = f1[j] = f2[j] = 0;
+        if (e < na) {
+            const int packed = rows[off + e];
+            const RowDesc ds = desc[packed & 0x3fffffff];
+            f0[j] = ds.f[0];
+            f1[j] = ds.f[1];
+            f2[j] = ds.f[2];
+            slot[j] = (unsigned)packed >> 30;
+        }
+    }
     const int64_t p0 = (int64_t)blockIdx.y * kchunk;
     int64_t p1 = p0 + kchunk;
     if (p1 > npts) p1 = npts;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
     for (int64_t pb = p0; pb < p1; pb += kGP) {
         __syncthreads();  // the previous batch is consumed
         for (int e = tid; e < nsel * kGP; e += 256) {
@@ -214,47 +240,54 @@ __global__ void __launch_bounds__(256)
             vv[tid] = V;
         }
         __syncthreads();
+        const int p = tc;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int idx = tid + 256 * j;  // 64 rows x 16 points
-            const int row = idx >> 4, p = idx & (kGP - 1);
-            const int rl = row & (kGT - 1);
-            const int e = ((row >> 5) ? tj : ti) * kGT + rl;
+        for (int j = 0; j < 8; ++j) {
             double u = 0.0, ph = 0.0;
-            if (e < na) {
-                const int packed = rows[off + e];
-                const RowDesc ds = desc[packed & 0x3fffffff];
-                const int slot = (unsigned)packed >> 30;
+            if (slot[j] >= 0) {
+                const uint32_t fs[3] = {f0[j], f1[j], f2[j]};
                 ph = 1.0;
                 u = 1.0;
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
-                    const int deg = ds.f[t] >> 16;
-                    const double q = deg ? qs[(ds.f[t] & 0xffff) * kGP + p] : 0.0;
+                    const int deg = fs[t] >> 16;
+                    const double q = deg ? qs[(fs[t] & 0xffff) * kGP + p] : 0.0;
                     const double h = hval(deg, q);
                     ph *= h;
-                    u *= (t == slot) ? eval_e(deg, q) : h;
+                    u *= (t == slot[j]) ? eval_e(deg, q) : h;
                 }
             }
-            if (row >> 5) {
-                uJ[rl][p] = u;
-                fJ[rl][p] = ph;
+            const int r = tr + 16 * (j & 3);
+            if (j < 4) {
+                aI[p][r] = wd[p] * u;
+                bI[p][r] = vv[p] * ph;
             } else {
-                uI[rl][p] = u;
-                fI[rl][p] = ph;
+                uJ[p][r] = u;
+                fJ[p][r] = ph;
             }
         }
         __syncthreads();
+#pragma unroll 4
+        for (int q = 0; q < kGP; ++q) {
+            double a[4], b[4], u[4], f[4];
 #pragma unroll
-        for (int p = 0; p < kGP; ++p) {
-            const double au = uI[r][p] * wd[p], af = fI[r][p] * vv[p];
+            for (int i = 0; i < 4; ++i) {
+                a[i] = aI[q][tr + 16 * i];
+                b[i] = bI[q][tr + 16 * i];
+                u[i] = uJ[q][tc + 16 * i];
+                f[i] = fJ[q][tc + 16 * i];
+            }
 #pragma unroll
-            for (int e = 0; e < 4; ++e) acc[e] += au * uJ[c0 + e][p] - af * fJ[c0 + e][p];
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * u[j] - b[i] * f[j];
         }
     }
-    double *dst = part + ((size_t)blockIdx.y * nwork + blockIdx.x) * (kGT * kGT) + r * kGT + c0;
+    double *dst = part + ((size_t)blockIdx.y * nwork + blockIdx.x) * (kGT * kGT);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) dst[e] = acc[e];
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dst[(tr + 16 * i) * kGT + tc + 16 * j] = acc[i][j];
 }
 
 // Sum the splits in order and add each element to H; two modes' blocks can

@@ -1,4 +1,4 @@
-// SCP3 command-line driver over libscp3g (the -G estimator on the GPU).
+// SCP3 command-line driver over libscp3g (both estimators on the GPU).
 //
 // Options and outputs follow SCP3.cpp:33-151,274-323:
 //   SCP3 [-N p] [-S p] [-r file] [-2 n] [-3 n] [-p pes] [-m list] [-G] [-B w] input
@@ -9,14 +9,16 @@
 //   -3/--triples n      modes with triple excitations (default 0)
 //   -p/--PES name       qtip4pf (default) or ttm3f
 //   -m/--select-modes   "a-b,c" list of modes (default all)
-//   -G/--no-gradient    the B^T diag(V) B estimator (required: the gradient
-//                       estimator has no GPU path)
+//   -G/--no-gradient    the B^T diag(V) B estimator; without it the gradient
+//                       (Stein-identity) estimator, on the GPU for qtip4pf
 //   -B/--block-width w  points per block between checkpoints (default 1024)
 // Additions: --precision fp64|mixed (default fp64), --levels k (lowest k
 // levels printed at the end, default 10, 0 = none), --device d, --cpu-pes
 // (the potential is evaluated on the host through h2o::PES, one instance
 // per thread, as the reference does; sampling, basis and SYRK stay on the
-// GPU: the path for potentials without a device kernel, such as WHBB).
+// GPU: the path for potentials without a device kernel, such as WHBB; -G
+// only), --stein (the gradient estimator with every element from the Stein
+// identity instead of the reference's placement of five element families).
 // Files: omega0.dat, E0_sd.dat (every 2^14 points), E0_t.dat (empty),
 // sM_%07d.h5 (every 2^17 points), levels.dat (this driver's final levels).
 #include <getopt.h>
@@ -39,7 +41,7 @@ struct Opts {
     int64_t skip = (int64_t)1 << 30;
     std::string rng_file, pes = "qtip4pf", input, modes_desc;
     int n2 = 0, n3 = 0, bw = 1024, levels = 10, device = 0;
-    bool gradient = true, have_modes = false, cpu_pes = false;
+    bool gradient = true, have_modes = false, cpu_pes = false, stein = false;
     int precision = SCP3G_FP64;
 };
 
@@ -83,6 +85,7 @@ Opts parse(int argc, char **argv)
                           {"levels", required_argument, nullptr, 1001},
                           {"device", required_argument, nullptr, 1002},
                           {"cpu-pes", no_argument, nullptr, 1003},
+                          {"stein", no_argument, nullptr, 1004},
                           {nullptr, 0, nullptr, 0}};
     Opts o;
     int ch;
@@ -118,6 +121,7 @@ Opts parse(int argc, char **argv)
         case 1001: o.levels = atoi(optarg); break;
         case 1002: o.device = atoi(optarg); break;
         case 1003: o.cpu_pes = true; break;
+        case 1004: o.stein = true; break;
         default: die("Unknown option: " + std::to_string(ch));
         }
     }
@@ -147,9 +151,13 @@ int main(int argc, char **argv)
     } else if (o.pes != "qtip4pf" && o.pes != "ttm3f") {
         die("Unknown H2O potential: " + o.pes);
     }
-    if (o.gradient)
-        die("The gradient estimator (the default without -G) has no GPU path in this build; "
-            "run with -G for the B^T diag(V) B estimator.");
+    if (o.gradient) {
+        // the gradient estimator needs dV/dx on the device (SCP3.cpp:210-217)
+        if (o.cpu_pes || o.pes != "qtip4pf")
+            die("The gradient estimator (the default without -G) runs on the GPU with qtip4pf only; "
+                "run with -G for the B^T diag(V) B estimator.");
+        if (o.precision != SCP3G_FP64) die("The gradient estimator runs in FP64 (--precision fp64).");
+    }
 
     cli::Input in;
     std::string err;
@@ -179,12 +187,15 @@ int main(int argc, char **argv)
     check(ctx, scp3g_set_basis(ctx, (int)modes.size(), modes.data(), o.n2, o.n3), "set_basis");
     if (!o.cpu_pes) check(ctx, scp3g_set_pes(ctx, o.pes.c_str()), "set_pes");
     check(ctx, scp3g_set_precision(ctx, o.precision), "set_precision");
+    if (o.gradient)
+        check(ctx, scp3g_set_estimator(ctx, o.stein ? SCP3G_EST_GRADIENT_STEIN : SCP3G_EST_GRADIENT),
+              "set_estimator");
     check(ctx, scp3g_reset(ctx), "reset");
     int64_t n = 0, n1 = 0, n2s = 0;
     scp3g_basis_size(ctx, &n);
     scp3g_sub_basis_size(ctx, 1, &n1);
     scp3g_sub_basis_size(ctx, 2, &n2s);
-    std::cerr << "SCP3 (GPU, -G): " << N / 3 << " waters, " << modes.size() << " modes, " << n
+    std::cerr << "SCP3 (GPU, " << (o.gradient ? (o.stein ? "gradient, Stein" : "gradient") : "-G") << "): " << N / 3 << " waters, " << modes.size() << " modes, " << n
               << " states, " << o.nsobol << " points, " << (o.precision ? "mixed" : "fp64")
               << " form, " << o.pes << "\n";
 

@@ -16,9 +16,36 @@ def run(args, cwd):
                           text=True, timeout=300)
 
 
-def test_without_G_says_so(tmp_path):
-    r = run(["SCP3", "-N", "10", DIMER], str(tmp_path))
-    assert r.returncode != 0 and "-G" in r.stderr
+def test_gradient_estimator_limits(tmp_path):
+    """Without -G the gradient estimator runs on the GPU with qtip4pf in FP64 only."""
+    for extra in (["-p", "ttm3f"], ["--cpu-pes"], ["--precision", "mixed"]):
+        r = run(["SCP3", "-N", "10"] + extra + [DIMER], str(tmp_path))
+        assert r.returncode != 0 and ("-G" in r.stderr or "FP64" in r.stderr), extra
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stein", [False, True])
+def test_scp3_gradient_run(tmp_path, stein):
+    """SCP3 without -G (the reference's default estimator) against the binding."""
+    import scp3_amd
+    from scp3_amd import system
+    print("cli case: dimer gradient -N 15 stein", stein, flush=True)
+    r = run(["SCP3", "-N", "15", "-2", "12", "--levels", "5"] + (["--stein"] if stein else []) + [DIMER],
+            str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    lv = np.loadtxt(tmp_path / "levels.dat")
+    e0 = np.loadtxt(tmp_path / "E0_sd.dat")
+    sysd = system.prepare(DIMER, nmodes2=12, nmodes3=0)
+    with scp3_amd.Context(0) as c:
+        c.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
+        c.set_basis(sysd["modes"], 12, 0)
+        c.set_pes("qtip4pf")
+        c.set_estimator(scp3_amd.EST_GRADIENT_STEIN if stein else scp3_amd.EST_GRADIENT)
+        c.accumulate(1 << 30, 1 << 15)
+        ev = c.lowest_levels(5)
+        e1 = c.lowest_levels_sub(c.sub_basis_size(1), 1)[0]
+    np.testing.assert_allclose(lv[:, 0], ev, rtol=0, atol=1e-12)
+    assert abs(e0[-1, 1] - e1 * scp3_amd.AUTOCM) < 1e-6
 
 
 def test_unknown_potential(tmp_path):
