@@ -106,8 +106,8 @@ int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast);
 /*
  * SCP3G_EST_ENERGY (default), SCP3G_EST_GRADIENT or SCP3G_EST_GRADIENT_STEIN
  * (SCP3.cpp:63-151 option -G selects the first; use_gradient the others).
- * Clears the running sum.  The gradient forms run in FP64 with the q-TIP4P/F
- * potential and need nmodes3 <= nmodes2; accumulate calls fail otherwise.
+ * Clears the running sum.  The gradient forms run in FP64 (q-TIP4P/F or
+ * TTM3-F) and need nmodes3 <= nmodes2; accumulate calls fail otherwise.
  */
 int scp3g_set_estimator(scp3g_ctx *ctx, int estimator);
 /* points per device tile (multiple of 256), or 0 (default): the largest power
@@ -251,7 +251,7 @@ int scp3g_pes_energy(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const void *
 /*
  * Energies and gradients of npts configurations (FP64): E_dev[i] as above and
  * G_dev[c*ldg + i] = dE/dx_c in kcal/mol/A, the gradient operator()
- * (PES.h:13-23; qtip4pf.cpp:252-397).  Only SCP3G_PES_QTIP4PF.
+ * (PES.h:13-23; qtip4pf.cpp:252-397, ttm3f.cpp:325-716).
  */
 int scp3g_pes_gradient(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const double *crd_dev,
                        int64_t ldc, double *E_dev, double *G_dev, int64_t ldg);

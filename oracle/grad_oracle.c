@@ -16,9 +16,9 @@
  * The oracle evaluates the rule directly with 1-D polynomial algebra per
  * mode (no factorisation), so it is independent of the GPU formulation.
  *
- * form 1 (SCP3G_GRAD_STEIN): every element from the rule; (0,0) = sum V
- * (ScaledMatrixElements.cpp:714).
- * form 0 (SCP3G_GRAD_REFERENCE): the reference's element placement, which
+ * form 1 (SCP3G_EST_GRADIENT_STEIN): every element from the rule; (0,0) = sum V
+ * (ScaledMatrixElements.cpp:717).
+ * form 0 (SCP3G_EST_GRADIENT): the reference's element placement, which
  * differs from the rule in five element families:
  *   column h1(k)h1(l), row h1(q_k): the rule for row h2(q_k)
  *       (v[k_index(m)] = m2_f_m1_l1(k, l), ScaledMatrixElements.cpp:195-199);
@@ -369,18 +369,22 @@ double orc_qtip4pf_grad(int64_t nw, const double *crd, double *grad)
 }
 
 /*
- * The SCP3_a block loop with use_gradient (SCP3.cpp:187-232) for the
- * q-TIP4P/F potential: y (seeds seed0.. or the caller's normals, halved),
+ * The SCP3_a block loop with use_gradient (SCP3.cpp:187-232), pes 0 =
+ * q-TIP4P/F, 1 = TTM3-F: y (seeds seed0.. or the caller's normals, halved),
  * r = bohr (x0 + MUa y), V and grad V, bV = V/autokcal - y.(omega o y)/2,
  * bVy = MUa^T grad V bohr/autokcal - omega o y, then the estimator over the
  * selected modes.  M (n x n, upper) gains the raw sums; V_out/bV_out/Vq_out
  * (npts x nsel) on request.
  */
+double orc_ttm3f_grad(int64_t nw, const double *crd, double *grad, double tol, int *iters)
+    __attribute__((weak));
+
 int orc_scp3_accumulate_grad(int nw, int nmodes0, const double *x0, const double *omega,
                              const double *MUa, int nsel, const int *modes, int nmodes2,
-                             int nmodes3, int form, int64_t seed0, int64_t npts,
+                             int nmodes3, int form, int pes, int64_t seed0, int64_t npts,
                              const double *normals, double *M, double *bV_out, double *Vq_out)
 {
+    if (pes == 1 && !orc_ttm3f_grad) return -4;
     const int n3 = 9 * nw;
     double *y = (double *)malloc(sizeof(double) * (size_t)nmodes0 * (size_t)npts);
     double *q = (double *)malloc(sizeof(double) * (size_t)(nsel ? nsel : 1) * (size_t)npts);
@@ -405,7 +409,8 @@ int orc_scp3_accumulate_grad(int nw, int nmodes0, const double *x0, const double
                 for (int k = 0; k < nmodes0; k++) s += MUa[(size_t)k * n3 + c] * yj[k];
                 r[c] = G_BOHR * (x0[c] + s);
             }
-            const double V = orc_qtip4pf_grad(nw, r, g) / G_AUTOKCALPMOL;
+            const double V = (pes == 1 ? orc_ttm3f_grad(nw, r, g, 1e-15, 0) : orc_qtip4pf_grad(nw, r, g)) /
+                             G_AUTOKCALPMOL;
             double h = 0.0;
             for (int k = 0; k < nmodes0; k++) h += yj[k] * (omega[k] * yj[k]);
             bV[j] = V - 0.5 * h;

@@ -61,7 +61,9 @@ def lib():
         L.orc_grad_accumulate.argtypes = [I, I, I, I, I64, P, P, P, P]
         L.orc_qtip4pf_grad.argtypes = [I64, P, P]
         L.orc_qtip4pf_grad.restype = D
-        L.orc_scp3_accumulate_grad.argtypes = [I, I, P, P, P, I, P, I, I, I, I64, I64, P, P, P, P]
+        L.orc_ttm3f_grad.argtypes = [I64, P, P, D, P]
+        L.orc_ttm3f_grad.restype = D
+        L.orc_scp3_accumulate_grad.argtypes = [I, I, P, P, P, I, P, I, I, I, I, I64, I64, P, P, P, P]
         if hasattr(L, "orc_ttm3f"):
             try:
                 L.orc_ttm3f.argtypes = [I64, P]
@@ -202,6 +204,14 @@ def qtip4pf_grad(crd):
     return e, g
 
 
+def ttm3f_grad(crd, tol=1e-15):
+    """TTM3-F energy (kcal/mol) and gradient (kcal/mol/A) of 9nw coordinates."""
+    c = np.ascontiguousarray(crd, dtype=np.float64).ravel()
+    g = np.zeros_like(c)
+    e = lib().orc_ttm3f_grad(c.size // 9, _p(c), _p(g), tol, None)
+    return e, g
+
+
 def grad_accumulate(q, V, vq, nsel, n2, n3, form=0):
     """Raw gradient-estimator sums (n x n Fortran, upper) for points q, V, vq.
 
@@ -218,8 +228,8 @@ def grad_accumulate(q, V, vq, nsel, n2, n3, form=0):
     return M
 
 
-def scp3_accumulate_grad(sysd, seed0, npts, form=0, normals=None):
-    """The gradient-estimator block loop (SCP3.cpp:187-232) with q-TIP4P/F.
+def scp3_accumulate_grad(sysd, seed0, npts, form=0, normals=None, pes="qtip4pf"):
+    """The gradient-estimator block loop (SCP3.cpp:187-232) with q-TIP4P/F or TTM3-F.
 
     Returns dict with the raw sums M (n x n Fortran, upper), bV and Vq
     (points x selected modes).
@@ -237,7 +247,8 @@ def scp3_accumulate_grad(sysd, seed0, npts, form=0, normals=None):
     MU = np.asfortranarray(sysd["MUa"], dtype=np.float64)
     nr = np.ascontiguousarray(normals, dtype=np.float64) if normals is not None else None
     rc = L.orc_scp3_accumulate_grad(nw, M0, _p(x0), _p(om), _p(MU), modes.size, _p(modes),
-                                    sysd["nmodes2"], sysd["nmodes3"], form, seed0, npts, _p(nr),
+                                    sysd["nmodes2"], sysd["nmodes3"], form, {"qtip4pf": 0, "ttm3f": 1}[pes],
+                                    seed0, npts, _p(nr),
                                     _p(Mm), _p(bV), _p(Vq))
     assert rc == 0, rc
     return dict(M=Mm, bV=bV, Vq=Vq)

@@ -538,9 +538,14 @@ static int run_tile_grad(scp3g_ctx *ctx, int64_t np)
     timer_end(ctx, "map", ev);
     if (rc) return rc;
     timer_begin(ctx, "pes", &ev);
-    rc = check_hip(ctx, launch_qtip4pf_grad(ctx, ctx->nw, np, (const double *)ctx->d_crd, T, ctx->d_E,
-                                            ctx->d_G, T),
-                   "pes gradient");
+    if (ctx->pes == SCP3G_PES_TTM3F)
+        rc = check_hip(ctx, launch_ttm3f_grad(ctx, ctx->nw, np, (const double *)ctx->d_crd, T, ctx->d_E,
+                                              ctx->d_G, T),
+                       "pes gradient");
+    else
+        rc = check_hip(ctx, launch_qtip4pf_grad(ctx, ctx->nw, np, (const double *)ctx->d_crd, T, ctx->d_E,
+                                                ctx->d_G, T),
+                       "pes gradient");
     timer_end(ctx, "pes", ev);
     if (rc) return rc;
     timer_begin(ctx, "weight", &ev);
@@ -599,8 +604,6 @@ int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
     if (ctx->estimator != SCP3G_EST_ENERGY) {
         if (wmode != 0) return fail(ctx, -4, "the gradient estimator has no surrogate or energy-input form");
         if (ctx->precision != SCP3G_FP64) return fail(ctx, -4, "the gradient estimator runs in FP64");
-        if (ctx->pes != SCP3G_PES_QTIP4PF)
-            return fail(ctx, -4, "the gradient estimator needs potential gradients on the GPU: qtip4pf only");
         if (ctx->nmodes3 > ctx->nmodes2) return fail(ctx, -4, "the gradient estimator needs nmodes3 <= nmodes2");
         return run_tile_grad(ctx, np);
     }
@@ -892,11 +895,13 @@ int scp3g_pes_gradient(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const doub
     if (!ctx) return -1;
     if (nw <= 0 || nw > 64 || npts < 0 || ldc < npts || ldg < npts || !crd_dev || !E_dev || !G_dev)
         return fail(ctx, -4, "bad arguments");
-    if (pes != SCP3G_PES_QTIP4PF) return fail(ctx, -4, "gradients on the GPU: qtip4pf only");
+    if (pes != SCP3G_PES_QTIP4PF && pes != SCP3G_PES_TTM3F) return fail(ctx, -4, "unknown potential id");
     hipSetDevice(ctx->device);
     hipEvent_t ev;
     timer_begin(ctx, "pes", &ev);
-    int rc = check_hip(ctx, launch_qtip4pf_grad(ctx, nw, npts, crd_dev, ldc, E_dev, G_dev, ldg), "qtip4pf");
+    int rc = pes == SCP3G_PES_TTM3F
+                 ? check_hip(ctx, launch_ttm3f_grad(ctx, nw, npts, crd_dev, ldc, E_dev, G_dev, ldg), "ttm3f")
+                 : check_hip(ctx, launch_qtip4pf_grad(ctx, nw, npts, crd_dev, ldc, E_dev, G_dev, ldg), "qtip4pf");
     timer_end(ctx, "pes", ev);
     return rc;
 }

@@ -462,6 +462,542 @@ hipError_t launch_ttm3f(scp3g_ctx *ctx, int nw, int64_t npts, const void *crd, i
     return hipGetLastError();
 }
 
+// ---- energies and gradients (ttm3f.cpp:325-716) --------------------------------
+
+// smear12 / smear123 without the potential term (ttm3f-bits-smear.cpp:142-190)
+__device__ __forceinline__ void smear123(double r12, double AA, double &ts1, double &ts2, double &ts3)
+{
+    const double dri = 1.0 / r12, drsqi = dri * dri;
+    const double rA = r12 / AA, rA3 = rA * rA * rA;
+    const double exp1 = exp(-ttm::aXX * rA3);
+    const double AA3 = AA * AA * AA;
+    ts1 = (1.0 - exp1) * dri * drsqi;
+    ts2 = (ts1 - exp1 * ttm::aXX / AA3) * drsqi;
+    ts3 = (ts2 - 0.6 * exp1 * r12 * ttm::aXX * ttm::aXX / (AA3 * AA3)) * drsqi;
+}
+
+// PS monomer energy (as ps_energy) and gradient dr[9] (ps.cpp:486-603), kcal/mol
+__device__ double ps_energy_grad(const double *O, const double *H1, const double *H2, double *dr)
+{
+    using namespace ttm;
+    double ROH1[3], ROH2[3], RHH[3], d1 = 0, d2 = 0, dh = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        ROH1[i] = H1[i] - O[i];
+        ROH2[i] = H2[i] - O[i];
+        RHH[i] = H1[i] - H2[i];
+        d1 += ROH1[i] * ROH1[i];
+        d2 += ROH2[i] * ROH2[i];
+        dh += RHH[i] * RHH[i];
+    }
+    d1 = sqrt(d1);
+    d2 = sqrt(d2);
+    dh = sqrt(dh);
+    const double costh = (ROH1[0] * ROH2[0] + ROH1[1] * ROH2[1] + ROH1[2] * ROH2[2]) / (d1 * d2);
+    const double deoh = f5z * deohA;
+    const double e1 = exp(-alphaoh * (d1 - roh));
+    const double e2 = exp(-alphaoh * (d2 - roh));
+    const double Va_shift = deoh * ((e1 - 1.0) * (e1 - 1.0) + (e2 - 1.0) * (e2 - 1.0));
+    const double Vb = f5z * phh1A * exp(phh2 * (1.0 - dh));
+    const double dVa1 = 2.0 * alphaoh * deoh * e1 * (1.0 - e1) / d1;
+    const double dVa2 = 2.0 * alphaoh * deoh * e2 * (1.0 - e2) / d2;
+    const double dVb = -phh2 * Vb / dh;
+    const double x1 = (d1 - reoh) / reoh, x2 = (d2 - reoh) / reoh, x3 = costh - costhe;
+    double f0[16], f1[16], f2[16];
+    f0[0] = f1[0] = f2[0] = 0.0;
+    f0[1] = f1[1] = f2[1] = 1.0;
+#pragma unroll
+    for (int j = 2; j < 16; ++j) {
+        f0[j] = f0[j - 1] * x1;
+        f1[j] = f1[j - 1] * x2;
+        f2[j] = f2[j - 1] * x3;
+    }
+    const double r1 = d1 - reoh, r2 = d2 - reoh;
+    const double efac = exp(-b1 * (r1 * r1 + r2 * r2));
+    double sum0 = 0, sum1 = 0, sum2 = 0, sum3 = 0;
+#pragma unroll
+    for (int j = 1; j < 245; ++j) {
+        const int inI = psdata::scp3_ps_idx1[j], inJ = psdata::scp3_ps_idx2[j], inK = psdata::scp3_ps_idx3[j];
+        const double c = psdata::scp3_ps_c5z[j];
+        const double s01 = f0[inI] * f1[inJ] + f0[inJ] * f1[inI];
+        sum0 += c * s01 * f2[inK];
+        sum1 += c * ((inI - 1) * f0[inI - 1] * f1[inJ] + (inJ - 1) * f0[inJ - 1] * f1[inI]) * f2[inK];
+        sum2 += c * ((inJ - 1) * f0[inI] * f1[inJ - 1] + (inI - 1) * f0[inJ] * f1[inI - 1]) * f2[inK];
+        sum3 += c * s01 * (inK - 1) * f2[inK - 1];
+    }
+    const double Kconst = 2.0 * psdata::scp3_ps_c5z[0] - 2.0 * deoh + 0.44739574026257;
+    const double dVcdr1 = (-2 * b1 * efac * r1 * sum0 + efac * sum1 / reoh) / d1;
+    const double dVcdr2 = (-2 * b1 * efac * r2 * sum0 + efac * sum2 / reoh) / d2;
+    const double dVcdcth = efac * sum3;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double gh1 = dVa1 * ROH1[i] + dVb * RHH[i] + dVcdr1 * ROH1[i] +
+                           dVcdcth * (ROH2[i] / (d1 * d2) - costh * ROH1[i] / (d1 * d1));
+        const double gh2 = dVa2 * ROH2[i] - dVb * RHH[i] + dVcdr2 * ROH2[i] +
+                           dVcdcth * (ROH1[i] / (d1 * d2) - costh * ROH2[i] / (d2 * d2));
+        dr[3 + i] = gh1 * cm2kcal;
+        dr[6 + i] = gh2 * cm2kcal;
+        dr[i] = -(gh1 + gh2) * cm2kcal;
+    }
+    return (Va_shift + Vb + efac * sum0 + Kconst) * cm2kcal;
+}
+
+// DMS charges (as dms_charges) and their derivatives, scaled as ttm3f.cpp:357-392:
+// grdq[k + 3(j + 3i)] = d charge_j / d atom_i along k, j = (H1, H2, M), i = (H1, H2, O)
+__device__ void dms_charges_grad(const double *O, const double *H1, const double *H2, double *chg,
+                                 double *grdq)
+{
+    using namespace ttm;
+    double ROH1[3], ROH2[3], d1 = 0, d2 = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        ROH1[i] = H1[i] - O[i];
+        ROH2[i] = H2[i] - O[i];
+        d1 += ROH1[i] * ROH1[i];
+        d2 += ROH2[i] * ROH2[i];
+    }
+    d1 = sqrt(d1);
+    d2 = sqrt(d2);
+    const double costh = (ROH1[0] * ROH2[0] + ROH1[1] * ROH2[1] + ROH1[2] * ROH2[2]) / (d1 * d2);
+    const double r1 = d1 - reoh, r2 = d2 - reoh;
+    const double efac = exp(-b1D * (r1 * r1 + r2 * r2));
+    const double x1 = r1 / reoh, x2 = r2 / reoh, x3 = costh - costhe;
+    double f0[16], f1[16], f2[16];
+    f0[0] = f1[0] = f2[0] = 0.0;
+    f0[1] = f1[1] = f2[1] = 1.0;
+#pragma unroll
+    for (int j = 2; j < 16; ++j) {
+        f0[j] = f0[j - 1] * x1;
+        f1[j] = f1[j - 1] * x2;
+        f2[j] = f2[j - 1] * x3;
+    }
+    double p1 = 0, p2 = 0, dp1dr1 = 0, dp1dr2 = 0, dp1dc = 0, dp2dr1 = 0, dp2dr2 = 0, dp2dc = 0;
+#pragma unroll
+    for (int j = 1; j < 84; ++j) {
+        const int inI = psdata::scp3_ps_idxD0[j], inJ = psdata::scp3_ps_idxD1[j], inK = psdata::scp3_ps_idxD2[j];
+        const double c = psdata::scp3_ps_coefD[j];
+        p1 += c * f0[inI] * f1[inJ] * f2[inK];
+        p2 += c * f0[inJ] * f1[inI] * f2[inK];
+        dp1dr1 += c * (inI - 1) * f0[inI - 1] * f1[inJ] * f2[inK];
+        dp1dr2 += c * (inJ - 1) * f0[inI] * f1[inJ - 1] * f2[inK];
+        dp1dc += c * (inK - 1) * f0[inI] * f1[inJ] * f2[inK - 1];
+        dp2dr1 += c * (inJ - 1) * f0[inJ - 1] * f1[inI] * f2[inK];
+        dp2dr2 += c * (inI - 1) * f0[inJ] * f1[inI - 1] * f2[inK];
+        dp2dc += c * (inK - 1) * f0[inJ] * f1[inI] * f2[inK - 1];
+    }
+    const double xx = 0.529177249, xx2 = xx * xx;
+    dp1dr1 /= reoh / xx;
+    dp1dr2 /= reoh / xx;
+    dp2dr1 /= reoh / xx;
+    dp2dr2 /= reoh / xx;
+    const double pl1 = costh, pl2 = 0.5 * (3.0 * pl1 * pl1 - 1.0);
+    const double ang_part = pc0 + pl1 * pc1 + pl2 * pc2;
+    const double pc = pa * (pow(d1, pb) + pow(d2, pb)) * ang_part;
+    const double dpc0dr1 = pa * pb * pow(d1, pb - 1) * ang_part * xx2;
+    const double dpc0dr2 = pa * pb * pow(d2, pb - 1) * ang_part * xx2;
+    const double dpc0dc = pa * (pow(d1, pb) + pow(d2, pb)) * (pc1 + 0.5 * (6.0 * pl1) * pc2) * xx;
+    const double defacdr1 = -2.0 * b1D * r1 * efac * xx;
+    const double defacdr2 = -2.0 * b1D * r2 * efac * xx;
+    dp1dr1 = dp1dr1 * efac + p1 * defacdr1 + dpc0dr1;
+    dp1dr2 = dp1dr2 * efac + p1 * defacdr2 + dpc0dr2;
+    dp1dc = dp1dc * efac + dpc0dc;
+    dp2dr1 = dp2dr1 * efac + p2 * defacdr1 + dpc0dr1;
+    dp2dr2 = dp2dr2 * efac + p2 * defacdr2 + dpc0dr2;
+    dp2dc = dp2dc * efac + dpc0dc;
+    p1 = psdata::scp3_ps_coefD[0] + p1 * efac + pc * xx;
+    p2 = psdata::scp3_ps_coefD[0] + p2 * efac + pc * xx;
+    double q0 = -(p1 + p2), q1 = p1, q2 = p2;
+    dp1dr1 /= xx;
+    dp1dr2 /= xx;
+    dp2dr1 /= xx;
+    dp2dr2 /= xx;
+    // TTM3 modification (ps.cpp:732-769)
+    const double ax = ROH1[1] * ROH2[2] - ROH1[2] * ROH2[1];
+    const double ay = -ROH1[0] * ROH2[2] + ROH1[2] * ROH2[0];
+    const double az = ROH1[0] * ROH2[1] - ROH1[1] * ROH2[0];
+    const double sinth = sqrt(ax * ax + ay * ay + az * az) / (d1 * d2);
+    const double ang = atan2(sinth, costh);
+    const double m1 = dms1 * (d1 - dms2) + dms3 * (ang - ath0);
+    const double m2 = dms1 * (d2 - dms2) + dms3 * (ang - ath0);
+    q0 = q0 - (m1 + m2);
+    q1 = q1 + m1;
+    q2 = q2 + m2;
+    dp1dr1 += dms1;
+    dp2dr2 += dms1;
+    dp1dc -= dms3 / sinth;
+    dp2dc -= dms3 / sinth;
+    // ps.cpp:773-826, then ttm3f.cpp:357-392
+    const double f1q1r13 = (dp1dr1 - (dp1dc * costh / d1)) / d1;
+    const double f1q1r23 = dp1dc / (d1 * d2);
+    const double f2q1r23 = (dp1dr2 - (dp1dc * costh / d2)) / d2;
+    const double f2q1r13 = dp1dc / (d2 * d1);
+    const double f1q2r13 = (dp2dr1 - (dp2dc * costh / d1)) / d1;
+    const double f1q2r23 = dp2dc / (d1 * d2);
+    const double f2q2r23 = (dp2dr2 - (dp2dc * costh / d2)) / d2;
+    const double f2q2r13 = dp2dc / (d2 * d1);
+    const double tmp = gamma2 / gamma1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double dq[3][3];  // [atom H1 H2 O][charge H1 H2 O]
+        dq[0][0] = f1q1r13 * ROH1[k] + f1q1r23 * ROH2[k];
+        dq[1][0] = f2q1r13 * ROH1[k] + f2q1r23 * ROH2[k];
+        dq[2][0] = -(dq[0][0] + dq[1][0]);
+        dq[0][1] = f1q2r13 * ROH1[k] + f1q2r23 * ROH2[k];
+        dq[1][1] = f2q2r13 * ROH1[k] + f2q2r23 * ROH2[k];
+        dq[2][1] = -(dq[0][1] + dq[1][1]);
+        dq[0][2] = -(dq[0][0] + dq[0][1]);
+        dq[1][2] = -(dq[1][0] + dq[1][1]);
+        dq[2][2] = -(dq[2][0] + dq[2][1]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            grdq[k + 3 * (0 + 3 * i)] = CHARGECON * (dq[i][0] + tmp * (dq[i][0] + dq[i][1]));
+            grdq[k + 3 * (1 + 3 * i)] = CHARGECON * (dq[i][1] + tmp * (dq[i][1] + dq[i][0]));
+            grdq[k + 3 * (2 + 3 * i)] = CHARGECON * (dq[i][2] - 2 * tmp * (dq[i][0] + dq[i][1]));
+        }
+    }
+    chg[0] = CHARGECON * (q0 / gamma1);
+    chg[1] = CHARGECON * (q1 + tmp * (q1 + q2));
+    chg[2] = CHARGECON * (q2 + tmp * (q1 + q2));
+}
+
+// Charge-charge pair terms of molecules i < j with their gradient: slot and
+// tens as pair_terms; gs[18] receives the direct forces on O_i H_i1 H_i2
+// O_j H_j1 H_j2 (ttm3f.cpp:397-486); returns the O-O term.
+__device__ double pair_terms_grad(const double *x, const double *ms, const double *q, int i, int j,
+                                  const SmearConst &sc, double *slot, double *tens, double *gs)
+{
+    using namespace ttm;
+    double ts0, ts1, ts2;
+#pragma unroll
+    for (int s = 0; s < 12; ++s) slot[s] = 0.0;
+#pragma unroll
+    for (int s = 0; s < 18; ++s) gs[s] = 0.0;
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {  // M(a) - H(b), ttm3f.cpp:403-430
+        const int a = dir ? j : i, b = dir ? i : j;
+        double *phiM = slot + (dir ? 3 : 0), *phiH = slot + (dir ? 0 : 3), *E = slot + (dir ? 9 : 6);
+        double *gH = gs + (dir ? 0 : 9);
+#pragma unroll
+        for (int l = 1; l < 3; ++l) {
+            double R[3], rsq = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                R[k] = ms[3 * a + k] - x[9 * b + 3 * l + k];
+                rsq += R[k] * R[k];
+            }
+            smear<double>(sqrt(rsq), sc.AA_HM, sc.a13, 1, ts0, ts1, ts2);
+            phiM[0] += ts0 * q[3 * b + l];
+            phiH[l] += ts0 * q[3 * a];
+            const double t1 = q[3 * b + l] * ts1, t2 = q[3 * a] * t1;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                E[k] += t1 * R[k];
+                gH[3 * l + k] += t2 * R[k];
+            }
+        }
+    }
+#pragma unroll
+    for (int ha = 1; ha < 3; ++ha)  // H-H, :439-463
+#pragma unroll
+        for (int hb = 1; hb < 3; ++hb) {
+            double R[3], rsq = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                R[k] = x[9 * i + 3 * ha + k] - x[9 * j + 3 * hb + k];
+                rsq += R[k] * R[k];
+            }
+            smear<double>(sqrt(rsq), sc.AA_HH, sc.a13, 1, ts0, ts1, ts2);
+            slot[ha] += q[3 * j + hb] * ts0;
+            slot[3 + hb] += q[3 * i + ha] * ts0;
+            const double qq = q[3 * i + ha] * q[3 * j + hb] * ts1;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                gs[3 * ha + k] -= qq * R[k];
+                gs[9 + 3 * hb + k] += qq * R[k];
+            }
+        }
+    double R[3], rsq = 0;  // O-O, :466-486
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        R[k] = x[9 * i + k] - x[9 * j + k];
+        rsq += R[k] * R[k];
+    }
+    const double r = sqrt(rsq), dR6 = rsq * rsq * rsq;
+    const double expon = vdwD * exp(-vdwE * r);
+    const double evdw = vdwC / dR6 + expon;
+    const double tv = -(6.0 * vdwC / dR6) / rsq - vdwE * expon / r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        gs[k] += tv * R[k];
+        gs[9 + k] -= tv * R[k];
+    }
+    rsq = 0;  // M-M, :488-535
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        R[k] = ms[3 * i + k] - ms[3 * j + k];
+        rsq += R[k] * R[k];
+    }
+    smear<double>(sqrt(rsq), sc.AA_MM, sc.a13, 2, ts0, ts1, ts2);
+    slot[0] += ts0 * q[3 * j];
+    slot[3] += ts0 * q[3 * i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        slot[6 + k] += q[3 * j] * ts1 * R[k];
+        slot[9 + k] -= q[3 * i] * ts1 * R[k];
+        tens[k] = R[k];
+    }
+    tens[3] = ts1;
+    tens[4] = ts2;
+    return evdw;
+}
+
+// Charge-dipole and dipole-dipole terms of molecules i < j, both directions
+// (ttm3f.cpp:596-668).  out[24]: dM_i, dM_j, forces on H_i1 H_i2, on H_j1
+// H_j2, potential changes at M_i H_i1 H_i2, at M_j H_j1 H_j2.
+__device__ void pair_dipole_grad(const double *x, const double *ms, const double *q, const double *dip,
+                                 int i, int j, const SmearConst &sc, double *out)
+{
+#pragma unroll
+    for (int s = 0; s < 24; ++s) out[s] = 0.0;
+    double ts1, ts2, ts3;
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {
+        const int a = dir ? j : i, b = dir ? i : j;
+        double *dMa = out + (dir ? 3 : 0), *dMb = out + (dir ? 0 : 3);
+        double *gHb = out + (dir ? 6 : 12), *phib = out + (dir ? 18 : 21);
+#pragma unroll
+        for (int l = 1; l < 3; ++l) {  // dipole of M(a) with the charge of H(b, l)
+            double R[3], rsq = 0, diR = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                R[k] = ms[3 * a + k] - x[9 * b + 3 * l + k];
+                rsq += R[k] * R[k];
+                diR += dip[3 * a + k] * R[k];
+            }
+            smear123(sqrt(rsq), sc.AA_HM, ts1, ts2, ts3);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double derij = q[3 * b + l] * (3 * ts2 * diR * R[k] - ts1 * dip[3 * a + k]);
+                dMa[k] += derij;
+                gHb[3 * (l - 1) + k] -= derij;
+            }
+            phib[l] -= ts1 * diR;
+        }
+        double R[3], rsq = 0, diR = 0;  // dipole of M(a) with the charge of M(b)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            R[k] = ms[3 * a + k] - ms[3 * b + k];
+            rsq += R[k] * R[k];
+            diR += dip[3 * a + k] * R[k];
+        }
+        smear123(sqrt(rsq), sc.AA_MM, ts1, ts2, ts3);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double derij = q[3 * b] * (3 * ts2 * diR * R[k] - ts1 * dip[3 * a + k]);
+            dMa[k] += derij;
+            dMb[k] -= derij;
+        }
+        phib[0] -= ts1 * diR;
+        if (dir == 0) {  // dipole-dipole, once per pair
+            double djR = 0, didj = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                djR += dip[3 * b + k] * R[k];
+                didj += dip[3 * a + k] * dip[3 * b + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double derij = -3 * ts2 * (didj * R[k] + djR * dip[3 * a + k] + diR * dip[3 * b + k]) +
+                                     15 * ts3 * diR * djR * R[k];
+                dMa[k] += derij;
+                dMb[k] -= derij;
+            }
+        }
+    }
+}
+
+// One configuration per wave, FP64; phases as k_ttm3f plus the gradient:
+// per-pair force slots summed per coordinate in a fixed partner order, the
+// dipole terms after the iteration, then the M-site distribution and the
+// charge-derivative terms per molecule.
+__global__ void __launch_bounds__(64)
+    k_ttm3f_grad(const double *__restrict__ crd, int64_t ldc, int nw, int64_t npts, double tol,
+                 SmearConst sc, double *__restrict__ E, double *__restrict__ G, int64_t ldg)
+{
+    using namespace ttm;
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x;
+    const int64_t cfg = blockIdx.x;
+    const int n3 = 3 * nw, P = nw * (nw - 1) / 2;
+    double *dipL = smem;              // [64]
+    double *x = smem + 64;            // [9nw]
+    double *ms = x + 9 * nw;          // [3nw]
+    double *q = ms + n3;              // [3nw]
+    double *g = q + n3;               // [9nw]
+    double *grdq = g + 9 * nw;        // [27nw]
+    double *phiS = grdq + 27 * nw;    // [3nw]
+    double *efqS = phiS + n3;         // [3nw]
+    double *dMS = efqS + n3;          // [3nw]
+    double *tens = dMS + n3;          // [P][5]
+    double *slots = tens + 5 * P;     // [P][12], then [P][18] forces; [P][24] after the iteration
+    double *gsl = slots + 12 * P;
+
+    for (int c = lane; c < 9 * nw; c += 64) x[c] = crd[(size_t)c * ldc + cfg];
+    __syncthreads();
+    double eint = 0.0;
+    if (lane < nw) {
+        const int w = lane;
+        const double *O = x + 9 * w, *H1 = O + 3, *H2 = O + 6;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ms[3 * w + k] = gamma1 * O[k] + gamma2 * (H1[k] + H2[k]);
+        eint = ps_energy_grad(O, H1, H2, g + 9 * w);
+        dms_charges_grad(O, H1, H2, q + 3 * w, grdq + 27 * w);
+    }
+    __syncthreads();
+    double evdw = 0.0;
+    for (int p = lane; p < P; p += 64) {
+        int i = 0, rem = p;
+        while (rem >= nw - 1 - i) {
+            rem -= nw - 1 - i;
+            ++i;
+        }
+        const int j = i + 1 + rem;
+        evdw += pair_terms_grad(x, ms, q, i, j, sc, slots + 12 * p, tens + 5 * p, gsl + 18 * p);
+    }
+    __syncthreads();
+    // forces of the charge-charge and O-O terms, per coordinate over partners
+    for (int c = lane; c < 9 * nw; c += 64) {
+        const int i = c / 9, e = c - 9 * i;
+        double s = 0.0;
+        for (int j = 0; j < nw; ++j) {
+            if (j == i) continue;
+            const bool first = i < j;
+            s += gsl[18 * pair_index(first ? i : j, first ? j : i, nw) + (first ? 0 : 9) + e];
+        }
+        g[c] += s;
+    }
+    double phi = 0.0, efq = 0.0;
+    if (lane < n3) {
+        const int i = lane / 3, t = lane - 3 * (lane / 3);
+        for (int j = 0; j < nw; ++j) {
+            if (j == i) continue;
+            const bool first = i < j;
+            const double *sl = slots + 12 * pair_index(first ? i : j, first ? j : i, nw);
+            phi += sl[(first ? 0 : 3) + t];
+            efq += sl[(first ? 6 : 9) + t];
+        }
+        phiS[lane] = phi;
+        efqS[lane] = efq;
+    }
+    const double eelec = 0.5 * wave_sum(lane < n3 ? phi * q[lane] : 0.0);
+    // induced dipoles, as k_ttm3f
+    const int ii = lane / 3, kk = lane - 3 * (lane / 3);
+    double dip = lane < n3 ? polarM * efq : 0.0;
+    double dip_prev = dip;
+    const double stath = DEBYE / CHARGECON / sqrt((double)nw);
+    for (int it = 0; it < maxiter; ++it) {
+        if (lane < n3) dipL[lane] = dip;
+        __syncthreads();
+        double efd = 0.0;
+        if (lane < n3) {
+            for (int j = 0; j < nw; ++j) {
+                if (j == ii) continue;
+                const double *tp = tens + 5 * pair_index(ii < j ? ii : j, ii < j ? j : ii, nw);
+                const double R0 = tp[0], R1 = tp[1], R2 = tp[2], t1 = tp[3], t2 = tp[4];
+                const double m0 = dipL[3 * j], m1 = dipL[3 * j + 1], m2 = dipL[3 * j + 2];
+                const double Rk = kk == 0 ? R0 : (kk == 1 ? R1 : R2);
+                const double mk = kk == 0 ? m0 : (kk == 1 ? m1 : m2);
+                efd += 3.0 * t2 * Rk * (R0 * m0 + R1 * m1 + R2 * m2) - t1 * mk;
+            }
+        }
+        double dn = lane < n3 ? polarM * (efq + efd) : 0.0;
+        dn = dmix * dn + (1.0 - dmix) * dip_prev;
+        const double delta = lane < n3 ? dn - dip_prev : 0.0;
+        const double dd = sqrt(wave_sum(delta * delta)) * stath;
+        dip = dn;
+        __syncthreads();
+        if (dd < tol) break;
+        dip_prev = dip;
+    }
+    const double eind = -0.5 * wave_sum(lane < n3 ? dip * efq : 0.0);
+    if (lane < n3) dipL[lane] = dip;
+    __syncthreads();
+    // charge-dipole and dipole-dipole terms per pair into [P][24]
+    for (int p = lane; p < P; p += 64) {
+        int i = 0, rem = p;
+        while (rem >= nw - 1 - i) {
+            rem -= nw - 1 - i;
+            ++i;
+        }
+        pair_dipole_grad(x, ms, q, dipL, i, i + 1 + rem, sc, slots + 24 * p);
+    }
+    __syncthreads();
+    // sums per molecule: forces on H (6nw), dM (3nw), potential (3nw)
+    for (int t = lane; t < 12 * nw; t += 64) {
+        int i, e, base_first, base_second;
+        if (t < 6 * nw) {
+            i = t / 6, e = t - 6 * i;
+            base_first = 6, base_second = 12;
+        } else if (t < 9 * nw) {
+            i = (t - 6 * nw) / 3, e = t - 6 * nw - 3 * i;
+            base_first = 0, base_second = 3;
+        } else {
+            i = (t - 9 * nw) / 3, e = t - 9 * nw - 3 * i;
+            base_first = 18, base_second = 21;
+        }
+        double s = 0.0;
+        for (int j = 0; j < nw; ++j) {
+            if (j == i) continue;
+            const bool first = i < j;
+            s += slots[24 * pair_index(first ? i : j, first ? j : i, nw) + (first ? base_first : base_second) + e];
+        }
+        if (t < 6 * nw) g[9 * i + 3 + e] += s;
+        else if (t < 9 * nw) dMS[3 * i + e] = s;
+        else phiS[3 * i + e] += s;
+    }
+    __syncthreads();
+    // M-site distribution (:672-686) and the charge derivatives (:690-713)
+    if (lane < nw) {
+        const int n = lane;
+        const double *gq = grdq + 27 * n;
+        const double pM = phiS[3 * n], pH1 = phiS[3 * n + 1], pH2 = phiS[3 * n + 2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double tmp = dMS[3 * n + k] - q[3 * n] * efqS[3 * n + k];
+            g[9 * n + k] += gamma1 * tmp + gq[k + 3 * (0 + 3 * 2)] * pH1 + gq[k + 3 * (1 + 3 * 2)] * pH2 +
+                            gq[k + 3 * (2 + 3 * 2)] * pM;
+            g[9 * n + 3 + k] += gamma2 * tmp + gq[k + 3 * (0 + 3 * 0)] * pH1 + gq[k + 3 * (1 + 3 * 0)] * pH2 +
+                                gq[k + 3 * (2 + 3 * 0)] * pM;
+            g[9 * n + 6 + k] += gamma2 * tmp + gq[k + 3 * (0 + 3 * 1)] * pH1 + gq[k + 3 * (1 + 3 * 1)] * pH2 +
+                                gq[k + 3 * (2 + 3 * 1)] * pM;
+        }
+    }
+    __syncthreads();
+    for (int c = lane; c < 9 * nw; c += 64) G[(size_t)c * ldg + cfg] = g[c];
+    const double ei = wave_sum(eint);
+    const double ev = wave_sum(evdw);
+    if (lane == 0) E[cfg] = ei + ev + eelec + eind;
+}
+
+size_t ttm3f_grad_smem_bytes(int nw)
+{
+    const int P = nw * (nw - 1) / 2;
+    const int pairs = 5 * P + (12 * P + 18 * P > 24 * P ? 12 * P + 18 * P : 24 * P);
+    return sizeof(double) * (size_t)(64 + 60 * nw + pairs);
+}
+
+hipError_t launch_ttm3f_grad(scp3g_ctx *ctx, int nw, int64_t npts, const double *crd, int64_t ldc,
+                             double *E, double *G, int64_t ldg)
+{
+    if (npts <= 0) return hipSuccess;
+    if (nw < 1 || 3 * nw > 64) return hipErrorInvalidValue;
+    const size_t smem = ttm3f_grad_smem_bytes(nw);
+    if (smem > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ttm3f_grad, dim3((unsigned)npts), dim3(64), smem, ctx->stream, crd, ldc, nw,
+                       npts, 1.0e-15, smear_const(), E, G, ldg);
+    return hipGetLastError();
+}
+
 // Host FP64 evaluation of the same terms in the same order as the kernel
 // (CPU potential for input preparation).
 double ttm3f_energy_host(int nw, const double *crd, int *iters)

@@ -10,7 +10,7 @@
 //   -p/--PES name       qtip4pf (default) or ttm3f
 //   -m/--select-modes   "a-b,c" list of modes (default all)
 //   -G/--no-gradient    the B^T diag(V) B estimator; without it the gradient
-//                       (Stein-identity) estimator, on the GPU for qtip4pf
+//                       (Stein-identity) estimator, on the GPU in FP64
 //   -B/--block-width w  points per block between checkpoints (default 1024)
 // Additions: --precision fp64|mixed (default fp64), --levels k (lowest k
 // levels printed at the end, default 10, 0 = none), --device d, --cpu-pes
@@ -153,9 +153,9 @@ int main(int argc, char **argv)
     }
     if (o.gradient) {
         // the gradient estimator needs dV/dx on the device (SCP3.cpp:210-217)
-        if (o.cpu_pes || o.pes != "qtip4pf")
-            die("The gradient estimator (the default without -G) runs on the GPU with qtip4pf only; "
-                "run with -G for the B^T diag(V) B estimator.");
+        if (o.cpu_pes)
+            die("The gradient estimator (the default without -G) needs the potential's gradient on the "
+                "GPU (qtip4pf, ttm3f); run --cpu-pes with -G.");
         if (o.precision != SCP3G_FP64) die("The gradient estimator runs in FP64 (--precision fp64).");
     }
 

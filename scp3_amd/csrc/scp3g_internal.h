@@ -176,6 +176,8 @@ void release_tiles128(scp3g_ctx *ctx);
 double qtip4pf_energy_host(int nw, const double *crd);
 hipError_t launch_qtip4pf_grad(scp3g_ctx *ctx, int nw, int64_t npts, const double *crd, int64_t ldc,
                                double *E, double *G, int64_t ldg);
+hipError_t launch_ttm3f_grad(scp3g_ctx *ctx, int nw, int64_t npts, const double *crd, int64_t ldc,
+                             double *E, double *G, int64_t ldg);
 // gradient estimator (grad.hip)
 int grad_setup(scp3g_ctx *ctx);
 void grad_release(scp3g_ctx *ctx);

@@ -130,8 +130,45 @@ def test_grad_estimator_rejects_unsupported(ctx, dimer):
     with pytest.raises(scp3_amd.Scp3Error, match="FP64"):
         ctx.accumulate(0, 256)
     ctx.set_precision(scp3_amd.FP64)
-    ctx.set_pes("ttm3f")
-    with pytest.raises(scp3_amd.Scp3Error, match="qtip4pf"):
-        ctx.accumulate(0, 256)
-    ctx.set_pes("qtip4pf")
+    with pytest.raises(scp3_amd.Scp3Error, match="energy-input"):
+        ctx.accumulate_energies(0, np.zeros(256))
     ctx.set_estimator(scp3_amd.EST_ENERGY)
+
+
+@pytest.mark.parametrize("name", ["hexamer_ttm3f", "w20_ttm3f"])
+def test_ttm3f_gradient_matches_oracle(ctx, name):
+    from scp3_amd import system
+    print("ttm3f gradient", name, flush=True)
+    sysd = system.prepare(system.input_path(name), nmodes2=0, nmodes3=0)
+    rng = np.random.default_rng(3)
+    n3 = sysd["x0"].size
+    nw = n3 // 9
+    P = 200
+    X = sysd["x0"][None, :] * scp3_amd.BOHR + rng.normal(size=(P, n3)) * 0.03
+    ld = 256
+    crd = torch.zeros(n3 * ld, dtype=torch.float64, device="cuda")
+    crd.view(n3, ld)[:, :P] = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    E = torch.zeros(ld, dtype=torch.float64, device="cuda")
+    G = torch.zeros(n3 * ld, dtype=torch.float64, device="cuda")
+    ctx.check(ctx.L.scp3g_pes_gradient(ctx.handle, scp3_amd.PES_TTM3F, nw, P, vp(crd), ld, vp(E), vp(G), ld))
+    ctx.sync()
+    Eg = E.cpu().numpy()[:P]
+    Gg = G.cpu().numpy().reshape(n3, ld)[:, :P].T
+    for i in range(0, P, 23):
+        e, g = oracle.ttm3f_grad(X[i])
+        assert Eg[i] == pytest.approx(e, rel=1e-12, abs=1e-10)
+        np.testing.assert_allclose(Gg[i], g, rtol=0, atol=1e-9 * np.max(np.abs(g)))
+
+
+def test_grad_sums_hexamer_ttm3f(ctx):
+    """The gradient estimator with TTM3-F (the oracle's TTM3-F gradient)."""
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path("hexamer_ttm3f"), nmodes2=12, nmodes3=6, modes=None)
+    setup(ctx, sysd, scp3_amd.EST_GRADIENT)
+    ctx.set_pes("ttm3f")
+    npts = 600
+    ctx.accumulate(1 << 30, npts)
+    M = ctx.h_download(raw=True)
+    ref = oracle.scp3_accumulate_grad(sysd, 1 << 30, npts, form=0, pes="ttm3f")["M"]
+    U = np.triu(np.ones(M.shape, bool))
+    assert np.max(np.abs((M - ref)[U])) <= 1e-9 * np.max(np.abs(ref[U]))

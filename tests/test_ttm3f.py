@@ -107,3 +107,39 @@ def test_gpu_ttm3f_matches_oracle(gpu_ctx_factory, name, nw):
             assert err < 5e-3
         else:
             np.testing.assert_allclose(Eg, ref, rtol=1e-12, atol=1e-11)
+
+
+# ---- gradients (ttm3f.cpp:325-716) -------------------------------------------
+
+def _ttm3f_geoms(name, count, scale, seed):
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path(name), nmodes2=0, nmodes3=0)
+    rng = np.random.default_rng(seed)
+    x0 = sysd["x0"] * scp3_amd.BOHR
+    return [x0 + rng.normal(size=x0.size) * scale for _ in range(count)]
+
+
+@pytest.mark.skipif(not oracle.has_ref(), reason="reference build absent")
+@pytest.mark.parametrize("name", ["hexamer_ttm3f", "w20_ttm3f"])
+def test_oracle_gradient_matches_reference(name):
+    R = oracle.ref()
+    for x in _ttm3f_geoms(name, 3, 0.03, 4):
+        x = np.ascontiguousarray(x)
+        g_ref = np.zeros_like(x)
+        e_ref = R.ref_ttm3f_grad(x.size // 9, ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(g_ref.ctypes.data))
+        e, g = oracle.ttm3f_grad(x)
+        assert e == pytest.approx(e_ref, rel=1e-12, abs=1e-10)
+        np.testing.assert_allclose(g, g_ref, rtol=0, atol=1e-9 * np.max(np.abs(g_ref)))
+
+
+def test_oracle_gradient_finite_differences():
+    x = _ttm3f_geoms("hexamer_ttm3f", 1, 0.03, 8)[0]
+    e, g = oracle.ttm3f_grad(x)
+    assert e == pytest.approx(oracle.ttm3f(x), rel=1e-12)
+    h = 1e-5
+    for i in range(0, x.size, 5):
+        xp, xm = x.copy(), x.copy()
+        xp[i] += h
+        xm[i] -= h
+        fd = (oracle.ttm3f(xp) - oracle.ttm3f(xm)) / (2 * h)
+        assert abs(fd - g[i]) < 2e-6 * max(1.0, np.max(np.abs(g))), i
