@@ -22,9 +22,11 @@ control variate: 2^24 surrogate + 2^19 true points against plain FP64 at
 plain FP64 at 2^20, lowest 20 levels).
 
 Forms: fp64, mixed, cv (above) and grad / stein: the gradient (Stein-
-identity) estimator, the reference's default without -G, in FP64 with the
-q-TIP4P/F potential (grad: the reference's element placement; stein: every
-element from the Stein identity), e.g. --preset c2 --forms grad.
+identity) estimator, the reference's default without -G (grad: the
+reference's element placement; stein: every element from the Stein
+identity), in FP64, or with the rank-2 term on split-BF16 MFMA
+(grad_mixed, stein_mixed), e.g. --preset c3 --forms stein,stein_mixed.
+The JSON line's main form is mixed, else cv, else the last form listed.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--preset c2]
 """
@@ -90,13 +92,13 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": n * 4, "check": float(dst[-1].item())}))
 
 
-GRAD_FORMS = ("grad", "stein")
+GRAD_FORMS = ("grad", "stein", "grad_mixed", "stein_mixed")
 
 
 def setup(c, sysd, pr, mode, form=None):
     import scp3_amd
-    c.set_estimator({"grad": scp3_amd.EST_GRADIENT, "stein": scp3_amd.EST_GRADIENT_STEIN}.get(
-        form, scp3_amd.EST_ENERGY))
+    est = {"grad": scp3_amd.EST_GRADIENT, "stein": scp3_amd.EST_GRADIENT_STEIN}
+    c.set_estimator(est.get((form or "").replace("_mixed", ""), scp3_amd.EST_ENERGY))
     c.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
     c.set_basis(sysd["modes"], sysd["nmodes2"], sysd["nmodes3"])
     c.set_pes("ttm3f" if "ttm3f" in pr["config"] else "qtip4pf")
@@ -108,7 +110,7 @@ def setup(c, sysd, pr, mode, form=None):
 
 def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
     import scp3_amd
-    mode = scp3_amd.FP64 if form in ("fp64",) + GRAD_FORMS else scp3_amd.MIXED
+    mode = scp3_amd.FP64 if form in ("fp64", "grad", "stein") else scp3_amd.MIXED
     setup(c, sysd, pr, mode, form)
     train_s = 0.0
     if form == "cv":
@@ -252,16 +254,17 @@ def main():
             print("bench: form %s accumulate %.4f s, levels %.4f s" % (f, res[f]["t_acc"], res[f]["t_lev"]),
                   file=sys.stderr, flush=True)
     n = res[forms[0]]["n"]
-    main_form = "mixed" if "mixed" in res else ("cv" if "cv" in res else forms[0])
+    main_form = "mixed" if "mixed" in res else ("cv" if "cv" in res else forms[-1])
     r = res[main_form]
     # dominant kernel: the SYRK's MFMA kernel (split-K partial fold timed
     # apart); algorithmic flops of a symmetric rank-K update: n(n+1) per point
     ms, cnt = r["kernels"]["syrk_mma"]
     avg_s = ms / 1e3 / max(cnt, 1)
     # points_done counts one step; the gradient forms run two SYRKs per tile
+    # (FP64) or one over twice the points (mixed)
     pts_per_launch = r["points"] * a.steps / max(cnt, 1) * (2 if main_form in GRAD_FORMS else 1)
     flops = float(n) * (n + 1) * pts_per_launch
-    fp64_main = main_form in ("fp64",) + GRAD_FORMS
+    fp64_main = main_form in ("fp64", "grad", "stein")
     peak = PEAK_FP64_TFLOPS if fp64_main else PEAK_BF16_TFLOPS
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
     out = {
@@ -301,6 +304,12 @@ def main():
         }
         if f == "cv":
             out["forms"][f].update(extra)
+    base = main_form.replace("_mixed", "")
+    if main_form.endswith("_mixed") and base in res:
+        d = np.abs(res[main_form]["levels"] - res[base]["levels"]).max() * scp3_amd.AUTOCM
+        out["max_level_diff_vs_%s_cm1" % base] = float(d)
+        out["speedup_time_to_levels_vs_%s" % base] = (out["forms"][base]["time_to_levels_s"]
+                                                      / out["forms"][main_form]["time_to_levels_s"])
     if "fp64" in res and main_form != "fp64":
         d = np.abs(res[main_form]["levels"] - res["fp64"]["levels"]).max() * scp3_amd.AUTOCM
         out["max_level_diff_vs_fp64_cm1"] = float(d)

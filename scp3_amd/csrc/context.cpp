@@ -181,6 +181,7 @@ static void free_sums(scp3g_ctx *ctx)
     dfree(ctx->d_H);
     dfree(ctx->d_Hc);
     dfree(ctx->d_F);
+    dfree(ctx->d_Heff);
     ctx->h_n = 0;
 }
 
@@ -222,7 +223,11 @@ static int ensure_work(scp3g_ctx *ctx)
     // the two BF16 planes of X (rows padded to 256) and the sign words, which
     // also hold the direct-load kernels' FP32 table (rows x T floats)
     const size_t rows_f = (size_t)fast_rows_pad(n);
-    const size_t bbytes = ctx->precision == SCP3G_FP64 ? sizeof(double) * rows * T : 4 * rows_f * T + T / 8 + 256;
+    // (the gradient estimator's mixed form holds the phi + s Psi and phi - s Psi
+    // planes side by side: twice the points)
+    const size_t bbytes = ctx->precision == SCP3G_FP64 ? sizeof(double) * rows * T
+                          : grad ? 8 * rows_f * T + T / 4 + 256
+                                 : 4 * rows_f * T + T / 8 + 256;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_B, bbytes), "alloc B"))) return rc;
     if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_B, 0, bbytes, ctx->stream), "clear B"))) return rc;
     if (grad) {
@@ -248,7 +253,7 @@ static int ensure_sums(scp3g_ctx *ctx)
     const bool grad = ctx->estimator != SCP3G_EST_ENERGY;
     int rc;
     if (grad && (rc = grad_setup(ctx))) return rc;
-    if (ctx->h_n == n && ctx->d_H && ctx->d_Hc && (!grad || ctx->d_F)) return 0;
+    if (ctx->h_n == n && ctx->d_H && ctx->d_Hc && (!grad || (ctx->d_F && ctx->d_Heff))) return 0;
     free_sums(ctx);
     const size_t bytes = sizeof(double) * (size_t)n * (size_t)n;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_H, bytes), "alloc H"))) return rc;
@@ -257,6 +262,7 @@ static int ensure_sums(scp3g_ctx *ctx)
     if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_Hc, 0, bytes, ctx->stream), "clear Hc")))
         return rc;
     if (grad) {
+        if ((rc = check_hip(ctx, hipMalloc(&ctx->d_Heff, bytes), "alloc Heff"))) return rc;
         if ((rc = check_hip(ctx, hipMalloc(&ctx->d_F, sizeof(double) * ctx->ngfix), "alloc F"))) return rc;
         if ((rc = check_hip(ctx, hipMemsetAsync(ctx->d_F, 0, sizeof(double) * ctx->ngfix, ctx->stream),
                             "clear F")))
@@ -556,6 +562,27 @@ static int run_tile_grad(scp3g_ctx *ctx, int64_t np)
                        "vq");
     timer_end(ctx, "weight", ev);
     if (rc) return rc;
+    if (ctx->precision == SCP3G_MIXED) {
+        // the pair phi +- s Psi as one signed split-BF16 SYRK over 2 Kh slots
+        const int64_t Kh = (np + 63) / 64 * 64;
+        const int64_t rows_pad = fast_rows_pad(ctx->nbasis);
+        unsigned short *Xh = reinterpret_cast<unsigned short *>(ctx->d_B);
+        unsigned short *Xl = Xh + (size_t)rows_pad * 2 * T;
+        unsigned *sgn = reinterpret_cast<unsigned *>(Xl + (size_t)rows_pad * 2 * T);
+        timer_begin(ctx, "basis", &ev);
+        rc = check_hip(ctx, launch_grad_split(ctx, ctx->d_y, T, np, Kh, ctx->d_w, ctx->d_Vq, T, ctx->d_sc,
+                                              ctx->d_wp, Xh, Xl, sgn),
+                       "grad split");
+        timer_end(ctx, "basis", ev);
+        if (rc) return rc;
+        timer_begin(ctx, "syrk", &ev);
+        rc = check_hip(ctx, launch_syrk_x3(ctx, ctx->nbasis, Xh, Xl, sgn, 2 * Kh, ctx->d_H, ctx->d_Hc, 3),
+                       "syrk");
+        timer_end(ctx, "syrk", ev);
+        if (rc) return rc;
+        goto corrections;
+    }
+    {
     double *B = (double *)ctx->d_B;
     for (int pass = 0; pass < 2; ++pass) {
         timer_begin(ctx, "basis", &ev);
@@ -571,6 +598,8 @@ static int run_tile_grad(scp3g_ctx *ctx, int64_t np)
         timer_end(ctx, "syrk", ev);
         if (rc) return rc;
     }
+    }
+corrections:
     timer_begin(ctx, "grad_blocks", &ev);
     rc = check_hip(ctx, launch_grad_corrections(ctx, ctx->d_y, T, np, ctx->d_w, ctx->d_Vq, T, ctx->d_H,
                                                 ctx->d_F),
@@ -590,8 +619,8 @@ int scp3g::read_sums(scp3g_ctx *ctx, const double **H)
     }
     int rc = ensure_sums(ctx);
     if (rc) return rc;
-    rc = check_hip(ctx, launch_grad_effective(ctx, ctx->d_H, ctx->d_F, ctx->d_Hc), "grad effective");
-    *H = ctx->d_Hc;
+    rc = check_hip(ctx, launch_grad_effective(ctx, ctx->d_H, ctx->d_F, ctx->d_Heff), "grad effective");
+    *H = ctx->d_Heff;
     return rc;
 }
 
@@ -603,7 +632,8 @@ int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
     int rc;
     if (ctx->estimator != SCP3G_EST_ENERGY) {
         if (wmode != 0) return fail(ctx, -4, "the gradient estimator has no surrogate or energy-input form");
-        if (ctx->precision != SCP3G_FP64) return fail(ctx, -4, "the gradient estimator runs in FP64");
+        if (ctx->precision == SCP3G_MIXED && !ctx->mixed_fast)
+            return fail(ctx, -4, "the gradient estimator's mixed form needs the LDS-tiled SYRK (mixed kernel 1 or 2)");
         if (ctx->nmodes3 > ctx->nmodes2) return fail(ctx, -4, "the gradient estimator needs nmodes3 <= nmodes2");
         return run_tile_grad(ctx, np);
     }

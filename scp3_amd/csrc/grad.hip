@@ -25,7 +25,7 @@
 //     s per point = 1/max(|V|, |V_d|) so both terms are O(phi);
 //   * the block corrections: for each mode d, the |A_d| x |A_d| upper
 //     triangle over the basis functions A_d that excite d (k_grad_blocks,
-//     FP64 VALU on 64 x 64 tiles);
+//     FP64 VALU on 64 x 64 or 128 x 128 tiles);
 //   * M(0,0) = sum V (ScaledMatrixElements.cpp:717) and the reference's own
 //     element placements (below) as a short list of per-point sums
 //     (k_grad_fix);
@@ -49,7 +49,6 @@ namespace scp3g {
 constexpr double kR2 = 1.4142135623730951;  // sqrt(2)
 constexpr double kR3 = 1.7320508075688772;  // sqrt(3)
 constexpr double kR6 = 2.4494897427831781;  // sqrt(6)
-constexpr int kGT = 64;                     // correction tile (rows of A_d)
 constexpr int kGP = 16;                     // points per staged batch
 constexpr int kFixPerThread = 24;           // per-point sums per thread of k_grad_fix
 
@@ -174,14 +173,18 @@ __global__ void __launch_bounds__(256)
 
 // ---- block corrections --------------------------------------------------------
 //
-// Work item (d, ti, tj): rows ti*64.. and columns tj*64.. (ti <= tj) of the
+// Work item (d, ti, tj): rows ti*TG.. and columns tj*TG.. (ti <= tj) of the
 // upper triangle of A_d; blockIdx.y takes a chunk of kchunk points.  Thread
-// (tr, tc) = (tid / 16, tid % 16) owns the 4 x 4 elements (tr + 16i,
-// tc + 16j).  Per batch of 16 points the block stages q (all selected
-// modes) and w_d, V; each thread then evaluates u^d and phi of the eight
-// rows tr + 16j (I rows for j < 4, J rows after) at point tc, storing
-// w_d u and V phi for the I side; the 16 points are then contracted from
-// LDS.  Partial sums go to part[split][work][64 x 64].
+// (tr, tc) = (tid / 16, tid % 16) owns the RT x RT elements (tr + 16i,
+// tc + 16j), RT = TG / 16.  Per batch of 16 points the block stages q (all
+// selected modes) and w_d, V; each thread then evaluates u^d and phi of the
+// rows tr + 16j (I rows for j < RT, J rows after) at point tc, storing
+// w_d u and V phi for the I side, and the batch is contracted from LDS.
+// TG = 64 suits the small A_d of a doubles-only basis (|A_d| = nmodes2 + 1),
+// TG = 128 the large ones of a basis with triples (fewer LDS reads per
+// FMA).  LDS (dynamic): q [nsel][16], row descriptors [2 TG], then
+// [16][TG + 1] x 4.  Partial sums go to part[split][work][TG x TG].
+template <int TG>
 __global__ void __launch_bounds__(256)
     k_grad_blocks(const double *__restrict__ y, int64_t ldy, const int *__restrict__ modes, int nsel,
                   const RowDesc *__restrict__ desc, const int *__restrict__ rows,
@@ -189,41 +192,36 @@ __global__ void __launch_bounds__(256)
                   const double *__restrict__ Vq, int64_t ldv, int64_t kchunk, int64_t npts,
                   double *__restrict__ part)
 {
-    extern __shared__ double qs[];  // [nsel][kGP]
-    // [point][row], rows padded by one double so that the staging stores of
-    // 16 points x 4 rows per wave fall in distinct banks
-    __shared__ double aI[kGP][kGT + 1], bI[kGP][kGT + 1], uJ[kGP][kGT + 1], fJ[kGP][kGT + 1];
+    constexpr int RT = TG / 16, LD = TG + 1;
+    extern __shared__ double gsm[];
+    double *qs = gsm;                                        // [nsel][kGP]
+    uint4 *rd = reinterpret_cast<uint4 *>(qs + nsel * kGP);  // [2 TG]: f0 f1 f2 slot
+    double *aI = reinterpret_cast<double *>(rd + 2 * TG);    // [kGP][LD] x 4
+    double *bI = aI + kGP * LD, *uJ = bI + kGP * LD, *fJ = uJ + kGP * LD;
     __shared__ double wd[kGP], vv[kGP];
     const int4 wk = work[blockIdx.x];  // {offset of A_d, |A_d|, d, ti | tj << 16}
     const int off = wk.x, na = wk.y, d = wk.z, ti = wk.w & 0xffff, tj = wk.w >> 16;
     const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
-    // this thread's staging rows: tr + 16j, I side for j < 4
-    uint32_t f0[8], f1[8], f2[8];
-    int slot[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int e = ((j < 4) ? ti : tj) * kGT + tr + 16 * (j & 3);
-        slot[j] = -1;
-        f0[j] = f1[j] = f2[j] = 0;
+    for (int r = tid; r < 2 * TG; r += 256) {
+        const int e = ((r < TG) ? ti : tj) * TG + (r & (TG - 1));
+        uint4 v = make_uint4(0, 0, 0, 0xffffffffu);
         if (e < na) {
             const int packed = rows[off + e];
             const RowDesc ds = desc[packed & 0x3fffffff];
-            f0[j] = ds.f[0];
-            f1[j] = ds.f[1];
-            f2[j] = ds.f[2];
-            slot[j] = (unsigned)packed >> 30;
+            v = make_uint4(ds.f[0], ds.f[1], ds.f[2], (unsigned)packed >> 30);
         }
+        rd[r] = v;
     }
     const int64_t p0 = (int64_t)blockIdx.y * kchunk;
     int64_t p1 = p0 + kchunk;
     if (p1 > npts) p1 = npts;
-    double acc[4][4];
+    double acc[RT][RT];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < RT; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+        for (int j = 0; j < RT; ++j) acc[i][j] = 0.0;
     for (int64_t pb = p0; pb < p1; pb += kGP) {
-        __syncthreads();  // the previous batch is consumed
+        __syncthreads();  // the previous batch is consumed (and rd is written)
         for (int e = tid; e < nsel * kGP; e += 256) {
             const int m = e / kGP, p = e - m * kGP;
             qs[e] = pb + p < p1 ? y[(size_t)modes[m] * ldy + pb + p] : 0.0;
@@ -241,11 +239,12 @@ __global__ void __launch_bounds__(256)
         }
         __syncthreads();
         const int p = tc;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < 2 * RT; ++j) {
+            const int r = tr + 16 * j;  // 0 .. 2 TG - 1
+            const uint4 ds = rd[r];
             double u = 0.0, ph = 0.0;
-            if (slot[j] >= 0) {
-                const uint32_t fs[3] = {f0[j], f1[j], f2[j]};
+            if (ds.w != 0xffffffffu) {
+                const uint32_t fs[3] = {ds.x, ds.y, ds.z};
                 ph = 1.0;
                 u = 1.0;
 #pragma unroll
@@ -254,57 +253,57 @@ __global__ void __launch_bounds__(256)
                     const double q = deg ? qs[(fs[t] & 0xffff) * kGP + p] : 0.0;
                     const double h = hval(deg, q);
                     ph *= h;
-                    u *= (t == slot[j]) ? eval_e(deg, q) : h;
+                    u *= (t == (int)ds.w) ? eval_e(deg, q) : h;
                 }
             }
-            const int r = tr + 16 * (j & 3);
-            if (j < 4) {
-                aI[p][r] = wd[p] * u;
-                bI[p][r] = vv[p] * ph;
+            if (r < TG) {
+                aI[p * LD + r] = wd[p] * u;
+                bI[p * LD + r] = vv[p] * ph;
             } else {
-                uJ[p][r] = u;
-                fJ[p][r] = ph;
+                uJ[p * LD + r - TG] = u;
+                fJ[p * LD + r - TG] = ph;
             }
         }
         __syncthreads();
-#pragma unroll 4
+#pragma unroll 2
         for (int q = 0; q < kGP; ++q) {
-            double a[4], b[4], u[4], f[4];
+            double a[RT], b[RT], u[RT], f[RT];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                a[i] = aI[q][tr + 16 * i];
-                b[i] = bI[q][tr + 16 * i];
-                u[i] = uJ[q][tc + 16 * i];
-                f[i] = fJ[q][tc + 16 * i];
+            for (int i = 0; i < RT; ++i) {
+                a[i] = aI[q * LD + tr + 16 * i];
+                b[i] = bI[q * LD + tr + 16 * i];
+                u[i] = uJ[q * LD + tc + 16 * i];
+                f[i] = fJ[q * LD + tc + 16 * i];
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < RT; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * u[j] - b[i] * f[j];
+                for (int j = 0; j < RT; ++j) acc[i][j] += a[i] * u[j] - b[i] * f[j];
         }
     }
-    double *dst = part + ((size_t)blockIdx.y * nwork + blockIdx.x) * (kGT * kGT);
+    double *dst = part + ((size_t)blockIdx.y * nwork + blockIdx.x) * (TG * TG);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < RT; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) dst[(tr + 16 * i) * kGT + tc + 16 * j] = acc[i][j];
+        for (int j = 0; j < RT; ++j) dst[(tr + 16 * i) * TG + tc + 16 * j] = acc[i][j];
 }
 
 // Sum the splits in order and add each element to H; two modes' blocks can
 // share an element (a, b exciting both), so the add is atomic.
 __global__ void __launch_bounds__(256)
     k_grad_blocks_reduce(const double *__restrict__ part, const int *__restrict__ rows,
-                         const int4 *__restrict__ work, int nwork, int nsplit, int64_t n,
+                         const int4 *__restrict__ work, int nwork, int nsplit, int tg, int64_t n,
                          double *__restrict__ H)
 {
     const int4 wk = work[blockIdx.x];
     const int off = wk.x, na = wk.y, ti = wk.w & 0xffff, tj = wk.w >> 16;
-    for (int e = threadIdx.x; e < kGT * kGT; e += 256) {
-        const int r = e / kGT, c = e - r * kGT;
-        const int er = ti * kGT + r, ec = tj * kGT + c;
+    const int tt = tg * tg;
+    for (int e = threadIdx.x; e < tt; e += 256) {
+        const int r = e / tg, c = e - r * tg;
+        const int er = ti * tg + r, ec = tj * tg + c;
         if (er >= na || ec >= na || (ti == tj && r > c)) continue;
         double s = 0.0;
-        for (int k = 0; k < nsplit; ++k) s += part[((size_t)k * nwork + blockIdx.x) * (kGT * kGT) + e];
+        for (int k = 0; k < nsplit; ++k) s += part[((size_t)k * nwork + blockIdx.x) * tt + e];
         const int64_t a = rows[off + er] & 0x3fffffff, b = rows[off + ec] & 0x3fffffff;
         atomicAdd(&H[(size_t)b * n + a], s);
     }
@@ -432,6 +431,21 @@ __global__ void k_grad_compat(const int4 *__restrict__ fix, int nfix, int nsel, 
     dst[(size_t)col * n + r2] = F[f];
 }
 
+// Mode d's block sum Hd (na x na, column-major upper, from the split-BF16
+// SYRK) added into H at the rows of A_d; the modes run one after another on
+// the stream, so the adds need no atomics.
+__global__ void __launch_bounds__(256)
+    k_corr_scatter(const double *__restrict__ Hd, int na, const int *__restrict__ rows, int off, int64_t n,
+                   double *__restrict__ H)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)na * na) return;
+    const int c = (int)(idx / na), r = (int)(idx - (int64_t)c * na);
+    if (r > c) return;
+    const int64_t a = rows[off + r] & 0x3fffffff, b = rows[off + c] & 0x3fffffff;
+    H[(size_t)b * n + a] += Hd[idx];
+}
+
 // ---- host -----------------------------------------------------------------------
 
 void grad_release(scp3g_ctx *ctx)
@@ -440,6 +454,11 @@ void grad_release(scp3g_ctx *ctx)
     if (ctx->d_gwork) hipFree(ctx->d_gwork);
     if (ctx->d_gfix) hipFree(ctx->d_gfix);
     if (ctx->d_gpart) hipFree(ctx->d_gpart);
+    if (ctx->d_gcx) hipFree(ctx->d_gcx);
+    ctx->d_gcx = nullptr;
+    ctx->gcx_bytes = 0;
+    ctx->gblock_off.clear();
+    ctx->gblock_na.clear();
     ctx->d_grows = nullptr;
     ctx->d_gwork = nullptr;
     ctx->d_gfix = nullptr;
@@ -465,10 +484,15 @@ int grad_setup(scp3g_ctx *ctx)
         }
     std::vector<int> rows;
     std::vector<int4> work;
+    size_t amax = 0;
+    for (const auto &v : A) amax = std::max(amax, v.size());
+    const int tg = amax > 64 ? 128 : 64;
     for (int d = 0; d < nsel; ++d) {
         const int off = (int)rows.size(), na = (int)A[d].size();
+        ctx->gblock_off.push_back(off);
+        ctx->gblock_na.push_back(na);
         rows.insert(rows.end(), A[d].begin(), A[d].end());
-        const int nt = (na + kGT - 1) / kGT;
+        const int nt = (na + tg - 1) / tg;
         for (int tj = 0; tj < nt; ++tj)
             for (int ti = 0; ti <= tj; ++ti) work.push_back(make_int4(off, na, d, ti | (tj << 16)));
     }
@@ -501,6 +525,7 @@ int grad_setup(scp3g_ctx *ctx)
     hipMemcpy(ctx->d_gwork, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice);
     hipMemcpy(ctx->d_gfix, fix.data(), fix.size() * sizeof(int4), hipMemcpyHostToDevice);
     ctx->ngwork = (int)work.size();
+    ctx->grad_tile = tg;
     ctx->ngfix = (int)fix.size();
     ctx->grad_n = n;
     ctx->grad_form = ctx->estimator;
@@ -566,27 +591,89 @@ hipError_t launch_grad_corrections(scp3g_ctx *ctx, const double *y, int64_t ldy,
     const int nsel = (int)ctx->modes.size();
     const int64_t n = ctx->nbasis;
     if ((size_t)2 * nsel * kGP * sizeof(double) + kGP * 8 > 64 * 1024) return hipErrorInvalidValue;
+    hipError_t e;
+    if (ctx->precision == SCP3G_MIXED && ctx->grad_tile == 128) {
+        // large A_d (a basis with triples): each mode's block as a signed
+        // split-BF16 SYRK over 2 Kh slots into a compact FP64 sum, then added
+        // into H.  The inner SYRK launches run untimed (the caller times the
+        // whole correction).
+        const int64_t Kh = (npts + 63) / 64 * 64, T = ctx->work_tile;
+        int namax = 0;
+        for (int v : ctx->gblock_na) namax = std::max(namax, v);
+        const int64_t rp = fast_rows_pad(namax);
+        const size_t plane = (size_t)rp * 2 * T * sizeof(unsigned short);
+        const size_t hd = (size_t)namax * namax * sizeof(double);
+        const size_t need = 2 * plane + (size_t)T / 4 + 256 + 2 * hd;
+        if (ctx->gcx_bytes < need) {
+            if (ctx->d_gcx) hipFree(ctx->d_gcx);
+            ctx->d_gcx = nullptr;
+            ctx->gcx_bytes = 0;
+            if ((e = hipMalloc(&ctx->d_gcx, need)) != hipSuccess) return e;
+            ctx->gcx_bytes = need;
+        }
+        char *base = reinterpret_cast<char *>(ctx->d_gcx);
+        unsigned short *Xh = reinterpret_cast<unsigned short *>(base);
+        unsigned short *Xl = reinterpret_cast<unsigned short *>(base + plane);
+        double *Hd = reinterpret_cast<double *>(base + 2 * plane);
+        double *Hcd = Hd + (size_t)namax * namax;
+        unsigned *sgn = reinterpret_cast<unsigned *>(Hcd + (size_t)namax * namax);
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        e = hipSuccess;
+        for (int d = 0; d < nsel && e == hipSuccess; ++d) {
+            const int na = ctx->gblock_na[d], off = ctx->gblock_off[d];
+            if (na == 0) continue;
+            const size_t bytes = (size_t)na * na * sizeof(double);
+            if ((e = hipMemsetAsync(Hd, 0, bytes, ctx->stream)) != hipSuccess) break;
+            if ((e = hipMemsetAsync(Hcd, 0, bytes, ctx->stream)) != hipSuccess) break;
+            if ((e = launch_corr_split(ctx, y, ldy, npts, Kh, d, ctx->d_grows, off, na, w, Vq, ldv, Xh, Xl,
+                                       sgn)) != hipSuccess)
+                break;
+            if ((e = launch_syrk_x3(ctx, na, Xh, Xl, sgn, 2 * Kh, Hd, Hcd, 3)) != hipSuccess) break;
+            hipLaunchKernelGGL(k_corr_scatter, dim3((unsigned)(((int64_t)na * na + 255) / 256)), dim3(256), 0,
+                               ctx->stream, Hd, na, ctx->d_grows, off, n, H);
+            e = hipGetLastError();
+        }
+        ctx->timing = timing;
+        if (e != hipSuccess) return e;
+        goto fixes;
+    }
+    {
     // blocks: split the points so the grid has >= ~2048 workgroups, chunks >= 1024 points
     int64_t nsplit = 1;
     while ((int64_t)ctx->ngwork * nsplit < 2048 && npts / (nsplit * 2) >= 1024) nsplit *= 2;
     int64_t kchunk = ((npts + nsplit - 1) / nsplit + kGP - 1) / kGP * kGP;
     nsplit = (npts + kchunk - 1) / kchunk;
+    const int tg = ctx->grad_tile;
+    const size_t need = (size_t)nsplit * ctx->ngwork * tg * tg * sizeof(double);
+    e = ensure_gpart(ctx, need);
+    if (e != hipSuccess) return e;
+    const size_t smem = (size_t)nsel * kGP * sizeof(double) + 2 * tg * sizeof(uint4) +
+                        (size_t)4 * kGP * (tg + 1) * sizeof(double);
+    if (smem > 150 * 1024) return hipErrorInvalidValue;
+    const void *kfn = tg == 128 ? reinterpret_cast<const void *>(k_grad_blocks<128>)
+                                : reinterpret_cast<const void *>(k_grad_blocks<64>);
+    if ((e = hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) != hipSuccess)
+        return e;
+    if (tg == 128)
+        hipLaunchKernelGGL(k_grad_blocks<128>, dim3((unsigned)ctx->ngwork, (unsigned)nsplit), dim3(256), smem,
+                           ctx->stream, y, ldy, ctx->d_modes, nsel, ctx->d_desc, ctx->d_grows, ctx->d_gwork,
+                           ctx->ngwork, w, Vq, ldv, kchunk, npts, ctx->d_gpart);
+    else
+        hipLaunchKernelGGL(k_grad_blocks<64>, dim3((unsigned)ctx->ngwork, (unsigned)nsplit), dim3(256), smem,
+                           ctx->stream, y, ldy, ctx->d_modes, nsel, ctx->d_desc, ctx->d_grows, ctx->d_gwork,
+                           ctx->ngwork, w, Vq, ldv, kchunk, npts, ctx->d_gpart);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_grad_blocks_reduce, dim3((unsigned)ctx->ngwork), dim3(256), 0, ctx->stream,
+                       ctx->d_gpart, ctx->d_grows, ctx->d_gwork, ctx->ngwork, (int)nsplit, tg, n, H);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+fixes:
     // fixes: one block per chunk of >= 2048 points, at most 512 blocks
     int64_t fsplit = std::min<int64_t>(512, std::max<int64_t>(1, npts / 2048));
     int64_t fchunk = ((npts + fsplit - 1) / fsplit + kGP - 1) / kGP * kGP;
     fsplit = (npts + fchunk - 1) / fchunk;
-    const size_t need = std::max((size_t)nsplit * ctx->ngwork * kGT * kGT, (size_t)fsplit * ctx->ngfix) *
-                        sizeof(double);
-    hipError_t e = ensure_gpart(ctx, need);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_grad_blocks, dim3((unsigned)ctx->ngwork, (unsigned)nsplit), dim3(256),
-                       (size_t)nsel * kGP * sizeof(double), ctx->stream, y, ldy, ctx->d_modes, nsel,
-                       ctx->d_desc, ctx->d_grows, ctx->d_gwork, ctx->ngwork, w, Vq, ldv, kchunk, npts,
-                       ctx->d_gpart);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_grad_blocks_reduce, dim3((unsigned)ctx->ngwork), dim3(256), 0, ctx->stream,
-                       ctx->d_gpart, ctx->d_grows, ctx->d_gwork, ctx->ngwork, (int)nsplit, n, H);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = ensure_gpart(ctx, (size_t)fsplit * ctx->ngfix * sizeof(double))) != hipSuccess) return e;
     hipLaunchKernelGGL(k_grad_fix, dim3((unsigned)fsplit), dim3(256),
                        ((size_t)2 * nsel * kGP + kGP) * sizeof(double), ctx->stream, ctx->d_gfix,
                        ctx->ngfix, y, ldy, ctx->d_modes, nsel, w, Vq, ldv, fchunk, npts, ctx->d_gpart);

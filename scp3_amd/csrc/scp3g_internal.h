@@ -99,11 +99,16 @@ struct scp3g_ctx {
     // gradient estimator: running sums of the fixed elements (ngfix), the
     // A_d row lists, correction work items and fix list (grad.hip)
     double *d_F = nullptr;
+    double *d_Heff = nullptr;  // the effective running sum, built when read (n x n)
     int *d_grows = nullptr;
     int4 *d_gwork = nullptr, *d_gfix = nullptr;
     double *d_gpart = nullptr;
     size_t gpart_bytes = 0;
-    int ngwork = 0, ngfix = 0, grad_form = -1;
+    // mixed form: per-mode block tables and the block's FP64 sum
+    void *d_gcx = nullptr;
+    size_t gcx_bytes = 0;
+    std::vector<int> gblock_off, gblock_na;  // A_d offset and size per mode
+    int ngwork = 0, ngfix = 0, grad_form = -1, grad_tile = 64;
     int64_t grad_n = 0;
     int64_t h_n = 0;
     int64_t points_done = 0;
@@ -191,6 +196,12 @@ hipError_t launch_grad_corrections(scp3g_ctx *ctx, const double *y, int64_t ldy,
                                    const double *w, const double *Vq, int64_t ldv, double *H,
                                    double *F);
 hipError_t launch_grad_effective(scp3g_ctx *ctx, const double *S, const double *F, double *dst);
+hipError_t launch_corr_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts, int64_t Kh, int d,
+                             const int *rows, int off, int na, const double *w, const double *Vq, int64_t ldv,
+                             unsigned short *Xh, unsigned short *Xl, unsigned *sgn);
+hipError_t launch_grad_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts, int64_t Kh,
+                             const double *w, const double *Vq, int64_t ldv, const double *sc,
+                             const double *wp, unsigned short *Xh, unsigned short *Xl, unsigned *sgn);
 // the running sum as the reference's M (before division by the points):
 // ctx->d_H for the energy estimator, built into ctx->d_Hc for the gradient one
 int read_sums(scp3g_ctx *ctx, const double **H);

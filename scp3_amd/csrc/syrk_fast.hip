@@ -31,6 +31,8 @@
 // partials.
 #include "scp3g_internal.h"
 
+#include <tuple>
+
 namespace scp3g {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -149,6 +151,257 @@ __global__ void __launch_bounds__(256)
         pl[a * 4] = make_uint4(ll[0] | (ll[1] << 16), ll[2] | (ll[3] << 16), ll[4] | (ll[5] << 16),
                                ll[6] | (ll[7] << 16));
     }
+}
+
+// The gradient estimator's symmetric rank-2 term Psi phi^T + phi Psi^T as
+// one signed SYRK over 2 Kh contraction slots: x = sqrt(wp) (phi + s Psi)
+// in k-blocks [0, Kh/32) with sign +, x = sqrt(wp) (phi - s Psi) in
+// [Kh/32, Kh/16) with sign -, wp = 1/(2s) (so wp [(phi+sPsi)(.)^T -
+// (phi-sPsi)(.)^T] = Psi phi^T + phi Psi^T).  Layout, threads and split as
+// k_basis_split; phase 1 puts h1..h3 and s * (s1..s3) of every selected mode
+// of the block's 64 points in LDS (the s factors of the Psi row, see grad.hip).
+__global__ void __launch_bounds__(256)
+    k_grad_split(const double *__restrict__ y, int64_t ldy, const int *__restrict__ modes, int nsel,
+                 const RowDesc *__restrict__ desc, int64_t n, int64_t rows_pad, int rows_per_block,
+                 int64_t npts, int64_t Kh, const double *__restrict__ w, const double *__restrict__ Vq,
+                 int64_t ldv, const double *__restrict__ sc, const double *__restrict__ wp,
+                 uint4 *__restrict__ Xh, uint4 *__restrict__ Xl)
+{
+    constexpr int PB = 64;
+    extern __shared__ float hsg[];  // [6][nsel][PB], then sqrt(wp) [PB]
+    float *sws = hsg + (size_t)6 * nsel * PB;
+    const int tid = threadIdx.x;
+    {
+        const int lane = tid & (PB - 1), grp = tid >> 6;
+        const int64_t i = (int64_t)blockIdx.x * PB + lane;
+        const bool live = i < npts;
+        for (int m = grp; m < nsel; m += 4) {
+            double q = 0.0, V = 0.0, vd = 0.0, t = 0.0;
+            if (live) {
+                q = y[(size_t)modes[m] * ldy + i];
+                V = w[i];
+                vd = Vq[(size_t)m * ldv + i];
+                t = sc[i];
+            }
+            hsg[(0 * nsel + m) * PB + lane] = (float)(1.4142135623730951 * q);
+            hsg[(1 * nsel + m) * PB + lane] = (float)((2.0 * q * q - 1.0) * 0.70710678118654752);
+            hsg[(2 * nsel + m) * PB + lane] = (float)((2.0 * q * q - 3.0) * q * 0.57735026918962576);
+            hsg[(3 * nsel + m) * PB + lane] = (float)(t * vd * 0.70710678118654752);
+            hsg[(4 * nsel + m) * PB + lane] = (float)(t * vd * q * 0.70710678118654752);
+            hsg[(5 * nsel + m) * PB + lane] =
+                (float)(t * (2.0 * V * q * 0.57735026918962576 + vd * (2.0 * q * q - 3.0) * 0.28867513459481288));
+        }
+        if (grp == 0) sws[lane] = live ? (float)sqrt(wp[i]) : 0.f;
+    }
+    __syncthreads();
+    const int half = tid >> 7, idx = tid & 127, rl = idx >> 2, pg = idx & 3;
+    const int p0 = half * 32 + pg * 8;
+    const int64_t kb = (int64_t)blockIdx.x * 2 + half;
+    if (kb * 32 >= Kh) return;
+    float sw[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sw[j] = sws[p0 + j];
+    const int64_t a0 = (int64_t)blockIdx.y * rows_per_block;
+    int64_t a1 = a0 + rows_per_block;
+    if (a1 > rows_pad) a1 = rows_pad;
+    const int64_t minus = Kh / 32 * rows_pad * 4;  // uint4 offset of the second half
+    uint4 *ph = Xh + (size_t)kb * rows_pad * 4 + pg;
+    uint4 *pl = Xl + (size_t)kb * rows_pad * 4 + pg;
+    for (int64_t a = a0 + rl; a < a1; a += 32) {
+        float vp[8], vm[8];
+        if (a < n) {
+            const RowDesc d = desc[a];
+            float f[3][8], g[3][8];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const int mode = d.f[t] & 0xffff, deg = d.f[t] >> 16;
+                if (deg) {
+                    const float4 *src = reinterpret_cast<const float4 *>(hsg + ((deg - 1) * nsel + mode) * PB + p0);
+                    const float4 *srs = reinterpret_cast<const float4 *>(hsg + ((deg + 2) * nsel + mode) * PB + p0);
+                    const float4 u0 = src[0], u1 = src[1], s0 = srs[0], s1 = srs[1];
+                    f[t][0] = u0.x; f[t][1] = u0.y; f[t][2] = u0.z; f[t][3] = u0.w;
+                    f[t][4] = u1.x; f[t][5] = u1.y; f[t][6] = u1.z; f[t][7] = u1.w;
+                    g[t][0] = s0.x; g[t][1] = s0.y; g[t][2] = s0.z; g[t][3] = s0.w;
+                    g[t][4] = s1.x; g[t][5] = s1.y; g[t][6] = s1.z; g[t][7] = s1.w;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        f[t][j] = 1.f;
+                        g[t][j] = 0.f;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float phi = (f[0][j] * f[1][j]) * f[2][j];
+                const float psi = g[0][j] * f[1][j] * f[2][j] + f[0][j] * g[1][j] * f[2][j] +
+                                  f[0][j] * f[1][j] * g[2][j];
+                vp[j] = sw[j] * (phi + psi);
+                vm[j] = sw[j] * (phi - psi);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) vp[j] = vm[j] = 0.f;
+        }
+        unsigned hh[8], ll[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            hh[j] = bf16_rne_u(vp[j]);
+            ll[j] = bf16_rne_u(vp[j] - __uint_as_float(hh[j] << 16));
+        }
+        ph[a * 4] = make_uint4(hh[0] | (hh[1] << 16), hh[2] | (hh[3] << 16), hh[4] | (hh[5] << 16),
+                               hh[6] | (hh[7] << 16));
+        pl[a * 4] = make_uint4(ll[0] | (ll[1] << 16), ll[2] | (ll[3] << 16), ll[4] | (ll[5] << 16),
+                               ll[6] | (ll[7] << 16));
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            hh[j] = bf16_rne_u(vm[j]);
+            ll[j] = bf16_rne_u(vm[j] - __uint_as_float(hh[j] << 16));
+        }
+        ph[minus + a * 4] = make_uint4(hh[0] | (hh[1] << 16), hh[2] | (hh[3] << 16), hh[4] | (hh[5] << 16),
+                                       hh[6] | (hh[7] << 16));
+        pl[minus + a * 4] = make_uint4(ll[0] | (ll[1] << 16), ll[2] | (ll[3] << 16), ll[4] | (ll[5] << 16),
+                                       ll[6] | (ll[7] << 16));
+    }
+}
+
+// The gradient estimator's block correction of mode d (see grad.hip) as a
+// signed SYRK over the rows A_d (the basis functions exciting d) and 2 Kh
+// slots: x = sqrt|w_d| u^d in k-blocks [0, Kh/32) with the sign of w_d =
+// V (4 q_d^2 + 2) - 2 q_d V_d, x = sqrt|V| phi in [Kh/32, Kh/16) with the
+// sign of -V.  rows[off + r] is the basis row of A_d's r-th member with the
+// factor slot of mode d in bits 30-31.  Layout and threads as k_basis_split.
+__global__ void __launch_bounds__(256)
+    k_corr_split(const double *__restrict__ y, int64_t ldy, const int *__restrict__ modes, int nsel,
+                 const RowDesc *__restrict__ desc, const int *__restrict__ rows, int off, int na,
+                 int64_t rows_pad, int rows_per_block, int64_t npts, int64_t Kh, int d,
+                 const double *__restrict__ w, const double *__restrict__ Vq, int64_t ldv,
+                 uint4 *__restrict__ Xh, uint4 *__restrict__ Xl)
+{
+    constexpr int PB = 64;
+    extern __shared__ float hsc[];  // [3][nsel][PB], e_1..3(q_d) [3][PB], sqrt|w_d|, sqrt|V| [2][PB]
+    float *es = hsc + (size_t)3 * nsel * PB, *su = es + 3 * PB, *sv = su + PB;
+    const int tid = threadIdx.x;
+    {
+        const int lane = tid & (PB - 1), grp = tid >> 6;
+        const int64_t i = (int64_t)blockIdx.x * PB + lane;
+        const bool live = i < npts;
+        for (int m = grp; m < nsel; m += 4) {
+            const double q = live ? y[(size_t)modes[m] * ldy + i] : 0.0;
+            hsc[(0 * nsel + m) * PB + lane] = (float)(1.4142135623730951 * q);
+            hsc[(1 * nsel + m) * PB + lane] = (float)((2.0 * q * q - 1.0) * 0.70710678118654752);
+            hsc[(2 * nsel + m) * PB + lane] = (float)((2.0 * q * q - 3.0) * q * 0.57735026918962576);
+        }
+        if (grp == 0) {
+            double qd = 0.0, V = 0.0, wd = 0.0;
+            if (live) {
+                qd = y[(size_t)modes[d] * ldy + i];
+                V = w[i];
+                wd = V * (4.0 * qd * qd + 2.0) - 2.0 * qd * Vq[(size_t)d * ldv + i];
+            }
+            es[lane] = 0.70710678118654752f;
+            es[PB + lane] = (float)(qd * 0.70710678118654752);
+            es[2 * PB + lane] = (float)((2.0 * qd * qd - 3.0) * 0.28867513459481288);
+            su[lane] = (float)sqrt(fabs(wd));
+            sv[lane] = (float)sqrt(fabs(V));
+        }
+    }
+    __syncthreads();
+    const int half = tid >> 7, idx = tid & 127, rl = idx >> 2, pg = idx & 3;
+    const int p0 = half * 32 + pg * 8;
+    const int64_t kb = (int64_t)blockIdx.x * 2 + half;
+    if (kb * 32 >= Kh) return;
+    const int64_t a0 = (int64_t)blockIdx.y * rows_per_block;
+    int64_t a1 = a0 + rows_per_block;
+    if (a1 > rows_pad) a1 = rows_pad;
+    const int64_t second = Kh / 32 * rows_pad * 4;
+    uint4 *ph = Xh + (size_t)kb * rows_pad * 4 + pg;
+    uint4 *pl = Xl + (size_t)kb * rows_pad * 4 + pg;
+    for (int64_t a = a0 + rl; a < a1; a += 32) {
+        float vu[8], vf[8];
+        if (a < na) {
+            const int packed = rows[off + a];
+            const RowDesc ds = desc[packed & 0x3fffffff];
+            const int slot = (unsigned)packed >> 30;
+            float u[8], f[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) u[j] = f[j] = 1.f;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const int mode = ds.f[t] & 0xffff, deg = ds.f[t] >> 16;
+                if (!deg) continue;
+                const float *hv = hsc + ((deg - 1) * nsel + mode) * PB + p0;
+                const float *ev = es + (deg - 1) * PB + p0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    f[j] *= hv[j];
+                    u[j] *= (t == slot) ? ev[j] : hv[j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                vu[j] = su[p0 + j] * u[j];
+                vf[j] = sv[p0 + j] * f[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) vu[j] = vf[j] = 0.f;
+        }
+        unsigned hh[8], ll[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            hh[j] = bf16_rne_u(vu[j]);
+            ll[j] = bf16_rne_u(vu[j] - __uint_as_float(hh[j] << 16));
+        }
+        ph[a * 4] = make_uint4(hh[0] | (hh[1] << 16), hh[2] | (hh[3] << 16), hh[4] | (hh[5] << 16),
+                               hh[6] | (hh[7] << 16));
+        pl[a * 4] = make_uint4(ll[0] | (ll[1] << 16), ll[2] | (ll[3] << 16), ll[4] | (ll[5] << 16),
+                               ll[6] | (ll[7] << 16));
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            hh[j] = bf16_rne_u(vf[j]);
+            ll[j] = bf16_rne_u(vf[j] - __uint_as_float(hh[j] << 16));
+        }
+        ph[second + a * 4] = make_uint4(hh[0] | (hh[1] << 16), hh[2] | (hh[3] << 16), hh[4] | (hh[5] << 16),
+                                        hh[6] | (hh[7] << 16));
+        pl[second + a * 4] = make_uint4(ll[0] | (ll[1] << 16), ll[2] | (ll[3] << 16), ll[4] | (ll[5] << 16),
+                                        ll[6] | (ll[7] << 16));
+    }
+}
+
+// Sign words of mode d's correction: w_d < 0 in the first Kh slots, V > 0 in
+// the next Kh (points beyond npts count as positive, their x is zero).
+__global__ void __launch_bounds__(256)
+    k_corr_sign_words(const double *__restrict__ y, int64_t ldy, const int *__restrict__ modes, int d,
+                      const double *__restrict__ w, const double *__restrict__ Vq, int64_t ldv,
+                      int64_t npts, int64_t Kh, unsigned *__restrict__ sgn)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool first = i < Kh;
+    const int64_t p = first ? i : i - Kh;
+    bool neg = false;
+    if (i < 2 * Kh && p < npts) {
+        const double V = w[p];
+        if (first) {
+            const double qd = y[(size_t)modes[d] * ldy + p];
+            neg = V * (4.0 * qd * qd + 2.0) - 2.0 * qd * Vq[(size_t)d * ldv + p] < 0.0;
+        } else {
+            neg = V > 0.0;
+        }
+    }
+    const unsigned long long m = __ballot(neg);
+    const int lane = threadIdx.x & 63;
+    if (i < 2 * Kh) {
+        if (lane == 0) sgn[i >> 5] = (unsigned)m;
+        if (lane == 32) sgn[i >> 5] = (unsigned)(m >> 32);
+    }
+}
+
+// Sign words of the pair: 0 for the first Kh/32 words, all ones after.
+__global__ void k_grad_sign_words(int64_t Kh, unsigned *__restrict__ sgn)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < Kh / 16) sgn[i] = i < Kh / 32 ? 0u : 0xffffffffu;
 }
 
 __device__ __forceinline__ void kahan_add2(double &h, double &c, double v)
@@ -584,18 +837,20 @@ struct WorkOrder {
     int4 *d = nullptr;
     int count = 0;
 };
-static std::map<scp3g_ctx *, WorkOrder> g_work;
+// one entry per (context, n, nsplit): the gradient estimator alternates
+// between the basis and its per-mode blocks
+static std::map<std::tuple<scp3g_ctx *, int64_t, int>, WorkOrder> g_work;
 
 struct FastTiles {
     int64_t n = -1;
     int2 *d = nullptr;
     int ntiles = 0;
 };
-static std::map<scp3g_ctx *, FastTiles> g_ftiles;
+static std::map<std::pair<scp3g_ctx *, int64_t>, FastTiles> g_ftiles;
 
 static hipError_t fast_tiles(scp3g_ctx *ctx, int64_t n, const int2 **tiles, int *ntiles)
 {
-    FastTiles &ft = g_ftiles[ctx];
+    FastTiles &ft = g_ftiles[std::make_pair(ctx, n)];
     if (ft.n != n) {
         if (ft.d) hipFree(ft.d);
         ft.d = nullptr;
@@ -617,15 +872,21 @@ static hipError_t fast_tiles(scp3g_ctx *ctx, int64_t n, const int2 **tiles, int 
 
 void release_fast_tiles(scp3g_ctx *ctx)
 {
-    auto iw = g_work.find(ctx);
-    if (iw != g_work.end()) {
-        if (iw->second.d) hipFree(iw->second.d);
-        g_work.erase(iw);
+    for (auto iw = g_work.begin(); iw != g_work.end();) {
+        if (std::get<0>(iw->first) == ctx) {
+            if (iw->second.d) hipFree(iw->second.d);
+            iw = g_work.erase(iw);
+        } else {
+            ++iw;
+        }
     }
-    auto it = g_ftiles.find(ctx);
-    if (it != g_ftiles.end()) {
-        if (it->second.d) hipFree(it->second.d);
-        g_ftiles.erase(it);
+    for (auto it = g_ftiles.begin(); it != g_ftiles.end();) {
+        if (it->first.first == ctx) {
+            if (it->second.d) hipFree(it->second.d);
+            it = g_ftiles.erase(it);
+        } else {
+            ++it;
+        }
     }
 }
 
@@ -655,6 +916,52 @@ hipError_t launch_basis_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int6
     return hipGetLastError();
 }
 
+hipError_t launch_grad_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts, int64_t Kh,
+                             const double *w, const double *Vq, int64_t ldv, const double *sc,
+                             const double *wp, unsigned short *Xh, unsigned short *Xl, unsigned *sgn)
+{
+    const int64_t n = ctx->nbasis, rows_pad = fast_rows_pad(n);
+    const int nsel = (int)ctx->modes.size();
+    const size_t smem = ((size_t)6 * nsel * 64 + 64) * sizeof(float);
+    if (smem > 160 * 1024 || Kh % 64) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_grad_split),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_grad_sign_words, dim3((unsigned)((Kh / 16 + 255) / 256)), dim3(256), 0, ctx->stream,
+                       Kh, sgn);
+    const int64_t pblocks = Kh / 64;
+    int64_t rpb = (rows_pad + 31) / 32 * 32;
+    while (rpb > 256 && pblocks * ((rows_pad + rpb - 1) / rpb) < 2048) rpb = (rpb / 2 + 31) / 32 * 32;
+    dim3 grid((unsigned)pblocks, (unsigned)((rows_pad + rpb - 1) / rpb));
+    hipLaunchKernelGGL(k_grad_split, grid, dim3(256), smem, ctx->stream, y, ldy, ctx->d_modes, nsel,
+                       ctx->d_desc, n, rows_pad, (int)rpb, npts, Kh, w, Vq, ldv, sc, wp,
+                       reinterpret_cast<uint4 *>(Xh), reinterpret_cast<uint4 *>(Xl));
+    return hipGetLastError();
+}
+
+hipError_t launch_corr_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts, int64_t Kh, int d,
+                             const int *rows, int off, int na, const double *w, const double *Vq, int64_t ldv,
+                             unsigned short *Xh, unsigned short *Xl, unsigned *sgn)
+{
+    const int64_t rows_pad = fast_rows_pad(na);
+    const int nsel = (int)ctx->modes.size();
+    const size_t smem = ((size_t)3 * nsel * 64 + 5 * 64) * sizeof(float);
+    if (smem > 160 * 1024 || Kh % 64) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_corr_split),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_corr_sign_words, dim3((unsigned)((2 * Kh + 255) / 256)), dim3(256), 0, ctx->stream, y,
+                       ldy, ctx->d_modes, d, w, Vq, ldv, npts, Kh, sgn);
+    const int64_t pblocks = Kh / 64;
+    int64_t rpb = (rows_pad + 31) / 32 * 32;
+    while (rpb > 256 && pblocks * ((rows_pad + rpb - 1) / rpb) < 2048) rpb = (rpb / 2 + 31) / 32 * 32;
+    dim3 grid((unsigned)pblocks, (unsigned)((rows_pad + rpb - 1) / rpb));
+    hipLaunchKernelGGL(k_corr_split, grid, dim3(256), smem, ctx->stream, y, ldy, ctx->d_modes, nsel, ctx->d_desc,
+                       rows, off, na, rows_pad, (int)rpb, npts, Kh, d, w, Vq, ldv, reinterpret_cast<uint4 *>(Xh),
+                       reinterpret_cast<uint4 *>(Xl));
+    return hipGetLastError();
+}
+
 // Dispatch order of the (tile, split) work items.  Consecutive workgroup
 // ids go to the 8 XCDs in turn and each XCD has its own L2.
 //  * split K (few tiles): the items are listed split-major (the tiles of one
@@ -671,7 +978,7 @@ hipError_t launch_basis_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int6
 
 static hipError_t work_order(scp3g_ctx *ctx, int64_t n, int nsplit, const int4 **work, int *count)
 {
-    WorkOrder &wo = g_work[ctx];
+    WorkOrder &wo = g_work[std::make_tuple(ctx, n, nsplit)];
     if (wo.n != n || wo.nsplit != nsplit) {
         if (wo.d) hipFree(wo.d);
         wo.d = nullptr;

@@ -127,8 +127,10 @@ def test_grad_normals_branch(ctx, dimer):
 def test_grad_estimator_rejects_unsupported(ctx, dimer):
     setup(ctx, dimer, scp3_amd.EST_GRADIENT)
     ctx.set_precision(scp3_amd.MIXED)
-    with pytest.raises(scp3_amd.Scp3Error, match="FP64"):
+    ctx.set_mixed_kernel(0)
+    with pytest.raises(scp3_amd.Scp3Error, match="LDS-tiled"):
         ctx.accumulate(0, 256)
+    ctx.set_mixed_kernel(1)
     ctx.set_precision(scp3_amd.FP64)
     with pytest.raises(scp3_amd.Scp3Error, match="energy-input"):
         ctx.accumulate_energies(0, np.zeros(256))
@@ -172,3 +174,43 @@ def test_grad_sums_hexamer_ttm3f(ctx):
     ref = oracle.scp3_accumulate_grad(sysd, 1 << 30, npts, form=0, pes="ttm3f")["M"]
     U = np.triu(np.ones(M.shape, bool))
     assert np.max(np.abs((M - ref)[U])) <= 1e-9 * np.max(np.abs(ref[U]))
+
+
+@pytest.mark.parametrize("est", [scp3_amd.EST_GRADIENT_STEIN, scp3_amd.EST_GRADIENT])
+def test_grad_mixed_matches_fp64(ctx, hexamer, est):
+    """The rank-2 term on split-BF16 MFMA against the FP64 form on the same points."""
+    P = 1 << 15
+    out = {}
+    for mode in (scp3_amd.FP64, scp3_amd.MIXED):
+        setup(ctx, hexamer, est, tile=16384)
+        ctx.set_precision(mode)
+        ctx.accumulate(1 << 30, P)
+        out[mode] = (ctx.h_download(), ctx.lowest_levels(10))
+    (H64, ev64), (Hmx, evmx) = out[scp3_amd.FP64], out[scp3_amd.MIXED]
+    dH = np.abs(np.triu(Hmx - H64)).max() / np.abs(np.triu(H64)).max()
+    dlev = np.abs(evmx - ev64).max() * scp3_amd.AUTOCM
+    print("gradient mixed vs fp64: max|dH|/max|H| %.3e, max|dlevel| %.4f cm^-1" % (dH, dlev), flush=True)
+    assert dH < 1e-5
+    assert dlev < 0.5
+    ctx.set_precision(scp3_amd.FP64)
+
+
+def test_grad_mixed_with_triples_matches_fp64(ctx):
+    """Blocks of more than 64 functions (triples): the per-mode corrections on
+    split-BF16 MFMA in the mixed form, against FP64 on the same points."""
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path("hexamer_qtip4pf"), nmodes2=48, nmodes3=12)
+    P = 1 << 14
+    out = {}
+    for mode in (scp3_amd.FP64, scp3_amd.MIXED):
+        setup(ctx, sysd, scp3_amd.EST_GRADIENT_STEIN, tile=8192)
+        ctx.set_precision(mode)
+        ctx.accumulate(1 << 30, P)
+        out[mode] = (ctx.h_download(), ctx.lowest_levels(10))
+    (H64, ev64), (Hmx, evmx) = out[scp3_amd.FP64], out[scp3_amd.MIXED]
+    dH = np.abs(np.triu(Hmx - H64)).max() / np.abs(np.triu(H64)).max()
+    dlev = np.abs(evmx - ev64).max() * scp3_amd.AUTOCM
+    print("gradient mixed (triples) vs fp64: max|dH|/max|H| %.3e, max|dlevel| %.4f cm^-1" % (dH, dlev), flush=True)
+    assert dH < 1e-5
+    assert dlev < 0.5
+    ctx.set_precision(scp3_amd.FP64)
