@@ -19,7 +19,8 @@ TTM3-F, 20825 states, 2^21 points, 32768-point tiles), c4 (c3 with the MLP
 control variate: 2^24 surrogate + 2^19 true points against plain FP64 at
 2^21), c5 (20-water cluster under TTM3-F, 174 modes, 15400 states, MLP
 174-512-512-512-1 on 2^19 pairs, 2^24 surrogate + 2^20 true points against
-plain FP64 at 2^20, lowest 20 levels).
+plain FP64 at 2^20, lowest 20 levels); c3g (c3 with the gradient
+estimator, Stein form, 2^20 points, FP64 against mixed).
 
 Forms: fp64, mixed, cv (above) and grad / stein: the gradient (Stein-
 identity) estimator, the reference's default without -G (grad: the
@@ -57,6 +58,8 @@ PRESETS = {
                cv_nn=24, cv_t=19, mlp_hidden=256, mlp_steps=20000, mlp_batch=4096, mlp_pairs=18),
     "c5": dict(config="w20_ttm3f", n2=174, n3=0, points=20, tile=0, levels=20, forms="fp64,cv",
                cv_nn=24, cv_t=20, mlp_hidden=512, mlp_steps=20000, mlp_batch=4096, mlp_pairs=19),
+    # c3 with the gradient (Stein) estimator: FP64 against the mixed form
+    "c3g": dict(config="hexamer_ttm3f", n2=48, n3=48, points=20, tile=0, levels=10, forms="stein,stein_mixed"),
 }
 
 

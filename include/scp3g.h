@@ -106,8 +106,10 @@ int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast);
 /*
  * SCP3G_EST_ENERGY (default), SCP3G_EST_GRADIENT or SCP3G_EST_GRADIENT_STEIN
  * (SCP3.cpp:63-151 option -G selects the first; use_gradient the others).
- * Clears the running sum.  The gradient forms run in FP64 (q-TIP4P/F or
- * TTM3-F) and need nmodes3 <= nmodes2; accumulate calls fail otherwise.
+ * Clears the running sum.  The gradient forms evaluate q-TIP4P/F or TTM3-F
+ * with gradients in FP64; in SCP3G_MIXED their rank-2 term and per-mode
+ * block corrections run as split-BF16 SYRKs (LDS-tiled kernel only).  They
+ * need nmodes3 <= nmodes2; accumulate calls fail otherwise.
  */
 int scp3g_set_estimator(scp3g_ctx *ctx, int estimator);
 /* points per device tile (multiple of 256), or 0 (default): the largest power

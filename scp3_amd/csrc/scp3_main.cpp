@@ -156,7 +156,6 @@ int main(int argc, char **argv)
         if (o.cpu_pes)
             die("The gradient estimator (the default without -G) needs the potential's gradient on the "
                 "GPU (qtip4pf, ttm3f); run --cpu-pes with -G.");
-        if (o.precision != SCP3G_FP64) die("The gradient estimator runs in FP64 (--precision fp64).");
     }
 
     cli::Input in;
