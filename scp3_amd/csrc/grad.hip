@@ -568,10 +568,15 @@ hipError_t launch_grad_table(scp3g_ctx *ctx, const double *y, int64_t ldy, int64
     const int64_t n = ctx->nbasis;
     const int64_t nrows_pad = (n + 63) / 64 * 64;
     const int nsel = (int)ctx->modes.size();
+    // points per block: the largest power of two <= 64 whose six value
+    // tables fit the LDS of a CU (160 KiB; 16 points for 174 modes)
     int pb_log2 = 6;
-    while (pb_log2 > 3 && (size_t)6 * nsel * (1u << pb_log2) * sizeof(double) > 64 * 1024) --pb_log2;
+    while (pb_log2 > 3 && (size_t)6 * nsel * (1u << pb_log2) * sizeof(double) > 160 * 1024) --pb_log2;
     const size_t smem = (size_t)6 * nsel * (1u << pb_log2) * sizeof(double);
-    if (smem > 64 * 1024) return hipErrorInvalidValue;
+    if (smem > 160 * 1024) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_grad_table),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
     const int64_t pblocks = (ldb + (1 << pb_log2) - 1) >> pb_log2;
     int rows_per_block = 256;
     while (rows_per_block > 16 && pblocks * ((nrows_pad + rows_per_block - 1) / rows_per_block) < 2048)

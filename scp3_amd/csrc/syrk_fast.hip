@@ -158,8 +158,10 @@ __global__ void __launch_bounds__(256)
 // in k-blocks [0, Kh/32) with sign +, x = sqrt(wp) (phi - s Psi) in
 // [Kh/32, Kh/16) with sign -, wp = 1/(2s) (so wp [(phi+sPsi)(.)^T -
 // (phi-sPsi)(.)^T] = Psi phi^T + phi Psi^T).  Layout, threads and split as
-// k_basis_split; phase 1 puts h1..h3 and s * (s1..s3) of every selected mode
-// of the block's 64 points in LDS (the s factors of the Psi row, see grad.hip).
+// k_basis_split; phase 1 puts q and s V_m of every selected mode and s V of
+// the block's 64 points in LDS (2 x 4 bytes per mode and point, so 174
+// modes fit), and each row's factors h_deg(q) and s-factors (see grad.hip)
+// are evaluated from them in FP32.
 __global__ void __launch_bounds__(256)
     k_grad_split(const double *__restrict__ y, int64_t ldy, const int *__restrict__ modes, int nsel,
                  const RowDesc *__restrict__ desc, int64_t n, int64_t rows_pad, int rows_per_block,
@@ -168,39 +170,34 @@ __global__ void __launch_bounds__(256)
                  uint4 *__restrict__ Xh, uint4 *__restrict__ Xl)
 {
     constexpr int PB = 64;
-    extern __shared__ float hsg[];  // [6][nsel][PB], then sqrt(wp) [PB]
-    float *sws = hsg + (size_t)6 * nsel * PB;
+    extern __shared__ float qsg[];  // q [nsel][PB], s V_m [nsel][PB], s V [PB], sqrt(wp) [PB]
+    float *tvs = qsg + (size_t)nsel * PB, *tV = tvs + (size_t)nsel * PB, *sws = tV + PB;
     const int tid = threadIdx.x;
     {
         const int lane = tid & (PB - 1), grp = tid >> 6;
         const int64_t i = (int64_t)blockIdx.x * PB + lane;
         const bool live = i < npts;
+        const double t = live ? sc[i] : 0.0;
         for (int m = grp; m < nsel; m += 4) {
-            double q = 0.0, V = 0.0, vd = 0.0, t = 0.0;
-            if (live) {
-                q = y[(size_t)modes[m] * ldy + i];
-                V = w[i];
-                vd = Vq[(size_t)m * ldv + i];
-                t = sc[i];
-            }
-            hsg[(0 * nsel + m) * PB + lane] = (float)(1.4142135623730951 * q);
-            hsg[(1 * nsel + m) * PB + lane] = (float)((2.0 * q * q - 1.0) * 0.70710678118654752);
-            hsg[(2 * nsel + m) * PB + lane] = (float)((2.0 * q * q - 3.0) * q * 0.57735026918962576);
-            hsg[(3 * nsel + m) * PB + lane] = (float)(t * vd * 0.70710678118654752);
-            hsg[(4 * nsel + m) * PB + lane] = (float)(t * vd * q * 0.70710678118654752);
-            hsg[(5 * nsel + m) * PB + lane] =
-                (float)(t * (2.0 * V * q * 0.57735026918962576 + vd * (2.0 * q * q - 3.0) * 0.28867513459481288));
+            qsg[m * PB + lane] = live ? (float)y[(size_t)modes[m] * ldy + i] : 0.f;
+            tvs[m * PB + lane] = live ? (float)(t * Vq[(size_t)m * ldv + i]) : 0.f;
         }
-        if (grp == 0) sws[lane] = live ? (float)sqrt(wp[i]) : 0.f;
+        if (grp == 0) {
+            tV[lane] = live ? (float)(t * w[i]) : 0.f;
+            sws[lane] = live ? (float)sqrt(wp[i]) : 0.f;
+        }
     }
     __syncthreads();
     const int half = tid >> 7, idx = tid & 127, rl = idx >> 2, pg = idx & 3;
     const int p0 = half * 32 + pg * 8;
     const int64_t kb = (int64_t)blockIdx.x * 2 + half;
     if (kb * 32 >= Kh) return;
-    float sw[8];
+    float sw[8], tv0[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) sw[j] = sws[p0 + j];
+    for (int j = 0; j < 8; ++j) {
+        sw[j] = sws[p0 + j];
+        tv0[j] = tV[p0 + j];
+    }
     const int64_t a0 = (int64_t)blockIdx.y * rows_per_block;
     int64_t a1 = a0 + rows_per_block;
     if (a1 > rows_pad) a1 = rows_pad;
@@ -215,17 +212,20 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
                 const int mode = d.f[t] & 0xffff, deg = d.f[t] >> 16;
-                if (deg) {
-                    const float4 *src = reinterpret_cast<const float4 *>(hsg + ((deg - 1) * nsel + mode) * PB + p0);
-                    const float4 *srs = reinterpret_cast<const float4 *>(hsg + ((deg + 2) * nsel + mode) * PB + p0);
-                    const float4 u0 = src[0], u1 = src[1], s0 = srs[0], s1 = srs[1];
-                    f[t][0] = u0.x; f[t][1] = u0.y; f[t][2] = u0.z; f[t][3] = u0.w;
-                    f[t][4] = u1.x; f[t][5] = u1.y; f[t][6] = u1.z; f[t][7] = u1.w;
-                    g[t][0] = s0.x; g[t][1] = s0.y; g[t][2] = s0.z; g[t][3] = s0.w;
-                    g[t][4] = s1.x; g[t][5] = s1.y; g[t][6] = s1.z; g[t][7] = s1.w;
-                } else {
+                const float *qm = qsg + mode * PB + p0, *vm_ = tvs + mode * PB + p0;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
+                for (int j = 0; j < 8; ++j) {
+                    const float q = deg ? qm[j] : 0.f, v = deg ? vm_[j] : 0.f;
+                    if (deg == 1) {
+                        f[t][j] = 1.41421356f * q;
+                        g[t][j] = 0.70710678f * v;
+                    } else if (deg == 2) {
+                        f[t][j] = (2.f * q * q - 1.f) * 0.70710678f;
+                        g[t][j] = 0.70710678f * v * q;
+                    } else if (deg == 3) {
+                        f[t][j] = (2.f * q * q - 3.f) * q * 0.57735027f;
+                        g[t][j] = 1.15470054f * tv0[j] * q + 0.28867513f * v * (2.f * q * q - 3.f);
+                    } else {
                         f[t][j] = 1.f;
                         g[t][j] = 0.f;
                     }
@@ -263,6 +263,13 @@ __global__ void __launch_bounds__(256)
         pl[minus + a * 4] = make_uint4(ll[0] | (ll[1] << 16), ll[2] | (ll[3] << 16), ll[4] | (ll[5] << 16),
                                        ll[6] | (ll[7] << 16));
     }
+}
+
+// Sign words of the pair: 0 for the first Kh/32 words, all ones after.
+__global__ void k_grad_sign_words(int64_t Kh, unsigned *__restrict__ sgn)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < Kh / 16) sgn[i] = i < Kh / 32 ? 0u : 0xffffffffu;
 }
 
 // The gradient estimator's block correction of mode d (see grad.hip) as a
@@ -395,13 +402,6 @@ __global__ void __launch_bounds__(256)
         if (lane == 0) sgn[i >> 5] = (unsigned)m;
         if (lane == 32) sgn[i >> 5] = (unsigned)(m >> 32);
     }
-}
-
-// Sign words of the pair: 0 for the first Kh/32 words, all ones after.
-__global__ void k_grad_sign_words(int64_t Kh, unsigned *__restrict__ sgn)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < Kh / 16) sgn[i] = i < Kh / 32 ? 0u : 0xffffffffu;
 }
 
 __device__ __forceinline__ void kahan_add2(double &h, double &c, double v)
@@ -922,7 +922,7 @@ hipError_t launch_grad_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int64
 {
     const int64_t n = ctx->nbasis, rows_pad = fast_rows_pad(n);
     const int nsel = (int)ctx->modes.size();
-    const size_t smem = ((size_t)6 * nsel * 64 + 64) * sizeof(float);
+    const size_t smem = ((size_t)2 * nsel * 64 + 2 * 64) * sizeof(float);
     if (smem > 160 * 1024 || Kh % 64) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_grad_split),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);

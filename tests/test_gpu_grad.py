@@ -214,3 +214,25 @@ def test_grad_mixed_with_triples_matches_fp64(ctx):
     assert dH < 1e-5
     assert dlev < 0.5
     ctx.set_precision(scp3_amd.FP64)
+
+
+def test_grad_mixed_w20_matches_fp64(ctx):
+    """174 selected modes (the 20-water cluster, TTM3-F): the rank-2 builder
+    with every mode in LDS, mixed against FP64 on the same points."""
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path("w20_ttm3f"), nmodes2=10, nmodes3=0)
+    P = 1 << 13
+    out = {}
+    for mode in (scp3_amd.FP64, scp3_amd.MIXED):
+        setup(ctx, sysd, scp3_amd.EST_GRADIENT_STEIN, tile=4096)
+        ctx.set_pes("ttm3f")
+        ctx.set_precision(mode)
+        ctx.accumulate(1 << 30, P)
+        out[mode] = (ctx.h_download(), ctx.lowest_levels(10))
+    (H64, ev64), (Hmx, evmx) = out[scp3_amd.FP64], out[scp3_amd.MIXED]
+    dH = np.abs(np.triu(Hmx - H64)).max() / np.abs(np.triu(H64)).max()
+    dlev = np.abs(evmx - ev64).max() * scp3_amd.AUTOCM
+    print("gradient mixed (w20) vs fp64: max|dH|/max|H| %.3e, max|dlevel| %.4f cm^-1" % (dH, dlev), flush=True)
+    assert dH < 1e-5
+    assert dlev < 0.5
+    ctx.set_precision(scp3_amd.FP64)
