@@ -313,8 +313,9 @@ __global__ void __launch_bounds__(256)
 //
 // fix = {type, k, l, col}: 0 = sum V (element (0,0)); 2 = m2_f_k1_l1 at
 // m = k, row h2(q_k) of column h1(k)h1(l); 4 = m2_f_k3 at m = k, row h2(q_k)
-// of column h3(k); 5 = V (4.5 q_k^2 - 3 q_k^4), the diagonal h3(k).  The
-// block holds all fixes (kFixPerThread per thread) for a chunk of points.
+// of column h3(k); 5 = V (4.5 q_k^2 - 3 q_k^4), the diagonal h3(k).  Block
+// (x, y) holds fixes from y * 256 kFixPerThread on (kFixPerThread per
+// thread) for chunk x of the points.
 __device__ __forceinline__ double fix_value(int type, double qk, double ql, double vk, double vl,
                                             double V)
 {
@@ -355,7 +356,7 @@ __global__ void __launch_bounds__(256)
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kFixPerThread; ++j) {
-            const int f = tid + 256 * j;
+            const int f = blockIdx.y * 256 * kFixPerThread + tid + 256 * j;
             if (f >= nfix) break;
             const int4 fx = fix[f];
             for (int p = 0; p < kGP; ++p)
@@ -365,7 +366,7 @@ __global__ void __launch_bounds__(256)
     }
 #pragma unroll
     for (int j = 0; j < kFixPerThread; ++j) {
-        const int f = tid + 256 * j;
+        const int f = blockIdx.y * 256 * kFixPerThread + tid + 256 * j;
         if (f < nfix) part[(size_t)blockIdx.x * nfix + f] = acc[j];
     }
 }
@@ -512,8 +513,6 @@ int grad_setup(scp3g_ctx *ctx)
             fix.push_back(make_int4(5, k, k, (int)(h30 + k)));
         }
     }
-    if ((int64_t)fix.size() > 256 * kFixPerThread)
-        return fail(ctx, -4, "gradient estimator: too many doubly excited modes (nmodes2 <= 77)");
     int rc;
     if ((rc = check_hip(ctx, hipMalloc(&ctx->d_grows, rows.size() * sizeof(int)), "alloc grad rows")))
         return rc;
@@ -679,7 +678,8 @@ fixes:
     int64_t fchunk = ((npts + fsplit - 1) / fsplit + kGP - 1) / kGP * kGP;
     fsplit = (npts + fchunk - 1) / fchunk;
     if ((e = ensure_gpart(ctx, (size_t)fsplit * ctx->ngfix * sizeof(double))) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_grad_fix, dim3((unsigned)fsplit), dim3(256),
+    const unsigned fy = (unsigned)((ctx->ngfix + 256 * kFixPerThread - 1) / (256 * kFixPerThread));
+    hipLaunchKernelGGL(k_grad_fix, dim3((unsigned)fsplit, fy), dim3(256),
                        ((size_t)2 * nsel * kGP + kGP) * sizeof(double), ctx->stream, ctx->d_gfix,
                        ctx->ngfix, y, ldy, ctx->d_modes, nsel, w, Vq, ldv, fchunk, npts, ctx->d_gpart);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -236,3 +236,29 @@ def test_grad_mixed_w20_matches_fp64(ctx):
     assert dH < 1e-5
     assert dlev < 0.5
     ctx.set_precision(scp3_amd.FP64)
+
+
+def test_reference_form_differs_only_in_five_families_large(ctx):
+    """80 doubly excited modes (6321 fixed elements, two blocks of fixes): the
+    reference's placement equals the Stein form outside the five families."""
+    import itertools
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path("w20_ttm3f"), nmodes2=80, nmodes3=0)
+    P = 2048
+    M = {}
+    for est in (scp3_amd.EST_GRADIENT, scp3_amd.EST_GRADIENT_STEIN):
+        setup(ctx, sysd, est, tile=2048)
+        ctx.set_pes("ttm3f")
+        ctx.accumulate(1 << 30, P)
+        M[est] = ctx.h_download(raw=True)
+    D = np.abs(np.triu(M[scp3_amd.EST_GRADIENT] - M[scp3_amd.EST_GRADIENT_STEIN]))
+    nsel, n2 = sysd["modes"].size, 80
+    fam = np.zeros(D.shape, bool)
+    c = 1 + nsel + n2
+    for k, l in itertools.combinations(range(n2), 2):
+        for m in (k, l):
+            fam[1 + m, c] = fam[1 + nsel + m, c] = True
+        c += 1
+    scale = np.abs(M[scp3_amd.EST_GRADIENT_STEIN]).max()
+    assert D[~fam].max() <= 1e-12 * scale
+    assert np.median(D[fam]) > 1e-9 * scale
