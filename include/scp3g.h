@@ -147,6 +147,15 @@ int scp3g_points_coords(scp3g_ctx *ctx, int64_t skip, int64_t npts, double *crd_
  * on the GPU (SCP3.cpp:213-231 with the PES call made by the caller).
  */
 int scp3g_accumulate_energies(scp3g_ctx *ctx, int64_t skip, int64_t npts, const double *E_host);
+/*
+ * The same for the gradient estimators (scp3g_set_estimator): the caller's
+ * energies E_host[npts] (kcal/mol) and gradients G_host[npts x 9nw]
+ * (kcal/mol/A, point-major, the gradient operator() of h2o::PES) for the
+ * Cartesians of scp3g_points_coords; bV, bVy (SCP3.cpp:210-223) and the
+ * estimator's SYRKs run on the GPU.
+ */
+int scp3g_accumulate_gradients(scp3g_ctx *ctx, int64_t skip, int64_t npts, const double *E_host,
+                               const double *G_host);
 
 /* Number of points accumulated since the last reset. */
 int scp3g_points_done(const scp3g_ctx *ctx, int64_t *npts);

@@ -54,6 +54,8 @@ _SIGS = {
     "scp3g_accumulate_surrogate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "scp3g_points_coords": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
     "scp3g_accumulate_energies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
+    "scp3g_accumulate_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                                  ctypes.c_void_p]),
     "scp3g_reset": (ctypes.c_int, [ctypes.c_void_p]),
     "scp3g_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "scp3g_accumulate_normals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
@@ -301,6 +303,13 @@ class Context:
         """accumulate Sobol seeds skip.. with caller energies E (kcal/mol)"""
         E = np.ascontiguousarray(E, dtype=np.float64)
         self.check(self.L.scp3g_accumulate_energies(self._h, int(skip), E.size, _ptr(E)))
+
+    def accumulate_gradients(self, skip, E, G):
+        """Caller's energies (kcal/mol) and gradients (points x 9nw, kcal/mol/A)
+        for the points of points_coords(skip, len(E)), into a gradient estimator."""
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        G = np.ascontiguousarray(G, dtype=np.float64)
+        self.check(self.L.scp3g_accumulate_gradients(self._h, int(skip), E.size, _ptr(E), _ptr(G)))
 
     def accumulate_surrogate(self, skip, npts):
         self.check(self.L.scp3g_accumulate_surrogate(self._h, int(skip), int(npts)))

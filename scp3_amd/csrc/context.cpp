@@ -534,11 +534,12 @@ int scp3g_reset(scp3g_ctx *ctx)
 // The gradient estimator's tile (SCP3.cpp:205-231 with use_gradient):
 // map, q-TIP4P/F energy and gradient, bV, Vq, the two FP64 SYRKs of
 // phi +- s Psi, and the block corrections and fixed elements.
-static int run_tile_grad(scp3g_ctx *ctx, int64_t np)
+static int run_tile_grad(scp3g_ctx *ctx, int64_t np, bool given)
 {
     const int64_t T = ctx->work_tile;
     hipEvent_t ev;
     int rc;
+    if (given) goto weights;  // the caller's energies and gradients are in d_E, d_G
     timer_begin(ctx, "map", &ev);
     rc = check_hip(ctx, launch_map(ctx, ctx->d_y, T, np, ctx->d_crd, T, 0), "map");
     timer_end(ctx, "map", ev);
@@ -554,6 +555,7 @@ static int run_tile_grad(scp3g_ctx *ctx, int64_t np)
                        "pes gradient");
     timer_end(ctx, "pes", ev);
     if (rc) return rc;
+weights:
     timer_begin(ctx, "weight", &ev);
     rc = check_hip(ctx, launch_weight(ctx, ctx->d_E, ctx->d_y, T, np, ctx->d_w), "weight");
     if (!rc)
@@ -631,11 +633,13 @@ int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
     hipEvent_t ev;
     int rc;
     if (ctx->estimator != SCP3G_EST_ENERGY) {
-        if (wmode != 0) return fail(ctx, -4, "the gradient estimator has no surrogate or energy-input form");
+        if (wmode != 0 && wmode != 4)
+            return fail(ctx, -4, "the gradient estimator has no surrogate or energy-input form "
+                                 "(scp3g_accumulate_gradients takes energies with gradients)");
         if (ctx->precision == SCP3G_MIXED && !ctx->mixed_fast)
             return fail(ctx, -4, "the gradient estimator's mixed form needs the LDS-tiled SYRK (mixed kernel 1 or 2)");
         if (ctx->nmodes3 > ctx->nmodes2) return fail(ctx, -4, "the gradient estimator needs nmodes3 <= nmodes2");
-        return run_tile_grad(ctx, np);
+        return run_tile_grad(ctx, np, wmode == 4);
     }
     if (wmode == 1) {
         timer_begin(ctx, "mlp", &ev);
@@ -809,6 +813,43 @@ int scp3g_accumulate_energies(scp3g_ctx *ctx, int64_t skip, int64_t npts, const 
                            "upload energies");
         if (!rc) rc = run_tile(ctx, np, 3, 1.0);
         // the host buffer is read asynchronously: finish before the next tile
+        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int scp3g_accumulate_gradients(scp3g_ctx *ctx, int64_t skip, int64_t npts, const double *E_host,
+                               const double *G_host)
+{
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (npts < 0 || skip < 0 || ((!E_host || !G_host) && npts > 0)) return fail(ctx, -4, "bad arguments");
+    if (ctx->estimator == SCP3G_EST_ENERGY)
+        return fail(ctx, -4, "scp3g_accumulate_gradients needs a gradient estimator (scp3g_set_estimator)");
+    if (skip + npts + 64 * 16 >= ((int64_t)1 << 60)) return fail(ctx, -4, "SEED too large");
+    hipSetDevice(ctx->device);
+    if ((rc = ensure_sums(ctx))) return rc;
+    if ((rc = ensure_work(ctx))) return rc;
+    const int64_t T = ctx->work_tile;
+    const int n3 = 9 * ctx->nw;
+    std::vector<double> gt((size_t)n3 * T);
+    for (int64_t done = 0; done < npts; done += T) {
+        const int64_t np = std::min<int64_t>(T, npts - done);
+        // point-major (9nw per point) -> [coordinate][point]
+        for (int64_t i = 0; i < np; ++i)
+            for (int c = 0; c < n3; ++c) gt[(size_t)c * T + i] = G_host[(size_t)(done + i) * n3 + c];
+        rc = check_hip(ctx, launch_sobol(ctx, ctx->d_sobol, ctx->nmodes0, skip + done, np, 1, ctx->d_y, T, nullptr),
+                       "sobol");
+        if (!rc)
+            rc = check_hip(ctx, hipMemcpyAsync(ctx->d_E, E_host + done, (size_t)np * 8, hipMemcpyHostToDevice,
+                                               ctx->stream),
+                           "upload energies");
+        if (!rc)
+            rc = check_hip(ctx, hipMemcpyAsync(ctx->d_G, gt.data(), gt.size() * 8, hipMemcpyHostToDevice, ctx->stream),
+                           "upload gradients");
+        if (!rc) rc = run_tile(ctx, np, 4, 1.0);
+        // the host buffers are read asynchronously: finish before the next tile
         if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
         if (rc) return rc;
     }

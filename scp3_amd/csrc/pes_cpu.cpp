@@ -2,8 +2,10 @@
 //
 // The energies are the same host/device functions the GPU kernels run
 // (qtip4pf_energy, ttm3f_energy) evaluated in FP64 on the host.  The
-// gradient overload uses central differences (step 1e-5 Angstrom); it serves
-// input preparation (relaxation, Hessians), not the hot path.
+// gradient overload is analytic for q-TIP4P/F (the kernel's function,
+// qtip4pf.cpp:252-397) and uses central differences (step 1e-5 Angstrom)
+// for TTM3-F, whose analytic gradient exists only as a device kernel; the
+// device path of the gradient estimator uses the analytic kernels.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -13,6 +15,7 @@
 
 namespace scp3g {
 double qtip4pf_energy_host(int nw, const double *crd);
+double qtip4pf_energy_grad_host(int nw, const double *crd, double *grad);
 double ttm3f_energy_host(int nw, const double *crd, int *iters);
 }  // namespace scp3g
 
@@ -29,6 +32,7 @@ public:
     }
     double operator()(size_t nw, const double *crd, double *grad) override
     {
+        if (kind_ == 0) return scp3g::qtip4pf_energy_grad_host((int)nw, crd, grad);
         const double h = 1e-5;
         std::vector<double> x(crd, crd + 9 * nw);
         for (size_t i = 0; i < 9 * nw; ++i) {

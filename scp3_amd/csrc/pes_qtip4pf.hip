@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(64)
 // gradient :84-103): coordinate c at me[c * st], dE/dx written to g[c * st]
 // (kcal/mol/A); mm and dm are scratch for the M sites and their gradient
 // (3nw each), distributed to O, H, H at the end as the reference does.
-__device__ double qtip4pf_energy_grad(const double *__restrict__ me, double *__restrict__ mm,
+__host__ __device__ double qtip4pf_energy_grad(const double *__restrict__ me, double *__restrict__ mm,
                                       double *__restrict__ g, double *__restrict__ dm, int st,
                                       int nw)
 {
@@ -306,6 +306,13 @@ hipError_t launch_qtip4pf_grad(scp3g_ctx *ctx, int nw, int64_t npts, const doubl
     hipLaunchKernelGGL(k_qtip4pf_grad, dim3((unsigned)((npts + L - 1) / L)), dim3(L), smem,
                        ctx->stream, crd, ldc, nw, npts, E, G, ldg);
     return hipGetLastError();
+}
+
+// host FP64 energy and gradient (the CPU h2o::PES gradient overload)
+double qtip4pf_energy_grad_host(int nw, const double *crd, double *grad)
+{
+    std::vector<double> mm((size_t)3 * nw), dm((size_t)3 * nw);
+    return qtip4pf_energy_grad(crd, mm.data(), grad, dm.data(), 1, nw);
 }
 
 double qtip4pf_energy_host(int nw, const double *crd)

@@ -16,8 +16,8 @@
 // levels printed at the end, default 10, 0 = none), --device d, --cpu-pes
 // (the potential is evaluated on the host through h2o::PES, one instance
 // per thread, as the reference does; sampling, basis and SYRK stay on the
-// GPU: the path for potentials without a device kernel, such as WHBB; -G
-// only), --stein (the gradient estimator with every element from the Stein
+// GPU: the path for potentials without a device kernel, such as WHBB; with
+// the PES's gradient overload without -G), --stein (the gradient estimator with every element from the Stein
 // identity instead of the reference's placement of five element families).
 // Files: omega0.dat, E0_sd.dat (every 2^14 points), E0_t.dat (empty),
 // sM_%07d.h5 (every 2^17 points), levels.dat (this driver's final levels).
@@ -151,12 +151,6 @@ int main(int argc, char **argv)
     } else if (o.pes != "qtip4pf" && o.pes != "ttm3f") {
         die("Unknown H2O potential: " + o.pes);
     }
-    if (o.gradient) {
-        // the gradient estimator needs dV/dx on the device (SCP3.cpp:210-217)
-        if (o.cpu_pes)
-            die("The gradient estimator (the default without -G) needs the potential's gradient on the "
-                "GPU (qtip4pf, ttm3f); run --cpu-pes with -G.");
-    }
 
     cli::Input in;
     std::string err;
@@ -217,17 +211,23 @@ int main(int argc, char **argv)
         } while (i + span < o.nsobol && (i + span) % log_every != 0);
         if (o.cpu_pes) {
             // SCP3.cpp:205-225 with the potential on the host
+            // (SCP3.cpp:210-217 with the gradient overload without -G)
             const int n3 = (int)(3 * N);
-            std::vector<double> crd((size_t)span * n3), E((size_t)span);
+            std::vector<double> crd((size_t)span * n3), E((size_t)span), G(o.gradient ? (size_t)span * n3 : 0);
             check(ctx, scp3g_points_coords(ctx, o.skip + i, span, crd.data()), "coords");
 #pragma omp parallel
             {
                 h2o::PES *pes = h2o::PESFromString(o.pes);
 #pragma omp for schedule(static)
-                for (int64_t j = 0; j < span; ++j) E[j] = (*pes)((size_t)(N / 3), crd.data() + (size_t)j * n3);
+                for (int64_t j = 0; j < span; ++j)
+                    E[j] = o.gradient ? (*pes)((size_t)(N / 3), crd.data() + (size_t)j * n3, G.data() + (size_t)j * n3)
+                                      : (*pes)((size_t)(N / 3), crd.data() + (size_t)j * n3);
                 delete pes;
             }
-            check(ctx, scp3g_accumulate_energies(ctx, o.skip + i, span, E.data()), "accumulate");
+            if (o.gradient)
+                check(ctx, scp3g_accumulate_gradients(ctx, o.skip + i, span, E.data(), G.data()), "accumulate");
+            else
+                check(ctx, scp3g_accumulate_energies(ctx, o.skip + i, span, E.data()), "accumulate");
         } else if (rng.is_open()) {
             std::vector<double> nr((size_t)span * nm0);
             for (double &v : nr) rng >> v;

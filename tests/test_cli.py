@@ -16,10 +16,24 @@ def run(args, cwd):
                           text=True, timeout=300)
 
 
-def test_gradient_estimator_limits(tmp_path):
-    """Without -G the gradient estimator needs a device potential (no --cpu-pes)."""
-    r = run(["SCP3", "-N", "10", "--cpu-pes", DIMER], str(tmp_path))
-    assert r.returncode != 0 and "-G" in r.stderr
+def test_gradient_estimator_unknown_cpu_potential(tmp_path):
+    r = run(["SCP3", "-N", "10", "--cpu-pes", "-p", "nonsense", DIMER], str(tmp_path))
+    assert r.returncode != 0 and "Unknown H2O potential" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cpu_pes_gradient_run_matches_device_run(tmp_path):
+    """--cpu-pes without -G: the gradient estimator from the host potential's
+    analytic gradient (q-TIP4P/F) equals the device-potential run."""
+    print("cli case: dimer gradient --cpu-pes -N 14", flush=True)
+    out = {}
+    for tag, extra in (("dev", []), ("cpu", ["--cpu-pes"])):
+        d = tmp_path / tag
+        d.mkdir()
+        r = run(["SCP3", "-N", "14", "-2", "12", "--levels", "5"] + extra + [DIMER], str(d))
+        assert r.returncode == 0, r.stderr
+        out[tag] = np.loadtxt(d / "levels.dat")
+    np.testing.assert_allclose(out["cpu"][:, 0], out["dev"][:, 0], rtol=0, atol=1e-10)
 
 
 @pytest.mark.gpu

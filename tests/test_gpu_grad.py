@@ -262,3 +262,27 @@ def test_reference_form_differs_only_in_five_families_large(ctx):
     scale = np.abs(M[scp3_amd.EST_GRADIENT_STEIN]).max()
     assert D[~fam].max() <= 1e-12 * scale
     assert np.median(D[fam]) > 1e-9 * scale
+
+
+@pytest.mark.parametrize("est", [scp3_amd.EST_GRADIENT, scp3_amd.EST_GRADIENT_STEIN])
+def test_accumulate_gradients_matches_device_potential(ctx, dimer, est):
+    """Energies and gradients from the caller (the oracle's q-TIP4P/F on the
+    Cartesians of points_coords) give the device path's sums."""
+    npts = 3000
+    setup(ctx, dimer, est, tile=2048)
+    X = ctx.points_coords(1 << 30, npts)
+    E = np.zeros(npts)
+    G = np.zeros_like(X)
+    for i in range(npts):
+        E[i], G[i] = oracle.qtip4pf_grad(X[i])
+    ctx.reset()
+    ctx.accumulate_gradients(1 << 30, E, G)
+    M1 = ctx.h_download(raw=True)
+    ctx.reset()
+    ctx.accumulate(1 << 30, npts)
+    M2 = ctx.h_download(raw=True)
+    U = np.triu(np.ones(M1.shape, bool))
+    assert np.max(np.abs((M1 - M2)[U])) <= 1e-9 * np.max(np.abs(M2[U]))
+    with pytest.raises(scp3_amd.Scp3Error, match="gradient estimator"):
+        ctx.set_estimator(scp3_amd.EST_ENERGY)
+        ctx.accumulate_gradients(1 << 30, E[:10], G[:10])

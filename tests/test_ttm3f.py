@@ -143,3 +143,15 @@ def test_oracle_gradient_finite_differences():
         xm[i] -= h
         fd = (oracle.ttm3f(xp) - oracle.ttm3f(xm)) / (2 * h)
         assert abs(fd - g[i]) < 2e-6 * max(1.0, np.max(np.abs(g))), i
+
+
+def test_cpu_pes_qtip4pf_gradient_is_analytic():
+    """The host h2o::PES gradient of q-TIP4P/F equals the oracle's analytic one."""
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path("hexamer_qtip4pf"), nmodes2=0, nmodes3=0)
+    rng = np.random.default_rng(12)
+    x = sysd["x0"] * scp3_amd.BOHR + rng.normal(size=sysd["x0"].size) * 0.03
+    e, g = scp3_amd.pes_cpu("qtip4pf", x, grad=True)
+    e0, g0 = oracle.qtip4pf_grad(x)
+    assert e == pytest.approx(e0, rel=1e-12)
+    np.testing.assert_allclose(g, g0, rtol=0, atol=1e-11 * np.max(np.abs(g0)))
