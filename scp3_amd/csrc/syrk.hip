@@ -15,6 +15,8 @@
 // lane reads 4 consecutive points (32 bytes) of a row per chunk.
 #include "scp3g_internal.h"
 
+#include <mutex>
+
 namespace scp3g {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -134,9 +136,12 @@ struct TileCache {
     int ntiles = 0;
 };
 static std::map<scp3g_ctx *, TileCache> g_tiles;
+// shared by all contexts (one per host thread): lookups and releases lock
+static std::mutex g_tiles_lock;
 
 static hipError_t get_tiles(scp3g_ctx *ctx, int64_t n, const int2 **tiles, int *ntiles)
 {
+    std::lock_guard<std::mutex> guard_(g_tiles_lock);
     TileCache &tc = g_tiles[ctx];
     if (tc.n != n) {
         if (tc.d_tiles) hipFree(tc.d_tiles);
@@ -161,6 +166,7 @@ hipError_t get_tiles_public(scp3g_ctx *ctx, int64_t n, const int2 **tiles, int *
 
 void release_tiles(scp3g_ctx *ctx)
 {
+    std::lock_guard<std::mutex> guard_(g_tiles_lock);
     auto it = g_tiles.find(ctx);
     if (it != g_tiles.end()) {
         if (it->second.d_tiles) hipFree(it->second.d_tiles);

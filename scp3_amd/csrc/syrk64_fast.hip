@@ -13,6 +13,8 @@
 // points are one 128-byte line.
 #include "scp3g_internal.h"
 
+#include <mutex>
+
 namespace scp3g {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -159,9 +161,12 @@ struct Tiles128 {
     int ntiles = 0;
 };
 static std::map<scp3g_ctx *, Tiles128> g_t128;
+// shared by all contexts (one per host thread): lookups and releases lock
+static std::mutex g_t128_lock;
 
 void release_tiles128(scp3g_ctx *ctx)
 {
+    std::lock_guard<std::mutex> guard_(g_t128_lock);
     auto it = g_t128.find(ctx);
     if (it != g_t128.end()) {
         if (it->second.d) hipFree(it->second.d);
@@ -176,6 +181,7 @@ int64_t rows_pad128(int64_t n) { return (n + kT64 - 1) / kT64 * kT64; }
 hipError_t launch_syrk_f64_lds(scp3g_ctx *ctx, int64_t n, const double *B, int64_t ldb,
                                const double *w, int64_t npts, double *H)
 {
+    std::lock_guard<std::mutex> guard_(g_t128_lock);
     Tiles128 &tc = g_t128[ctx];
     hipError_t e;
     if (tc.n != n) {

@@ -31,6 +31,7 @@
 // partials.
 #include "scp3g_internal.h"
 
+#include <mutex>
 #include <tuple>
 
 namespace scp3g {
@@ -847,9 +848,12 @@ struct FastTiles {
     int ntiles = 0;
 };
 static std::map<std::pair<scp3g_ctx *, int64_t>, FastTiles> g_ftiles;
+// shared by all contexts (one per host thread): lookups and releases lock
+static std::mutex g_fast_lock;
 
 static hipError_t fast_tiles(scp3g_ctx *ctx, int64_t n, const int2 **tiles, int *ntiles)
 {
+    std::lock_guard<std::mutex> guard_(g_fast_lock);
     FastTiles &ft = g_ftiles[std::make_pair(ctx, n)];
     if (ft.n != n) {
         if (ft.d) hipFree(ft.d);
@@ -872,6 +876,7 @@ static hipError_t fast_tiles(scp3g_ctx *ctx, int64_t n, const int2 **tiles, int 
 
 void release_fast_tiles(scp3g_ctx *ctx)
 {
+    std::lock_guard<std::mutex> guard_(g_fast_lock);
     for (auto iw = g_work.begin(); iw != g_work.end();) {
         if (std::get<0>(iw->first) == ctx) {
             if (iw->second.d) hipFree(iw->second.d);
@@ -978,6 +983,7 @@ hipError_t launch_corr_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int64
 
 static hipError_t work_order(scp3g_ctx *ctx, int64_t n, int nsplit, const int4 **work, int *count)
 {
+    std::lock_guard<std::mutex> guard_(g_fast_lock);
     WorkOrder &wo = g_work[std::make_tuple(ctx, n, nsplit)];
     if (wo.n != n || wo.nsplit != nsplit) {
         if (wo.d) hipFree(wo.d);

@@ -387,3 +387,36 @@ def test_cpu_potential_path_matches_device_path(ctx, dimer):
     H_dev = ctx.h_download(raw=True)
     mx, fro = _rel(H_cpu, H_dev)
     assert mx < 1e-11 and fro < 1e-11, (mx, fro)
+
+
+def test_two_contexts_in_two_threads(dimer, hexamer):
+    """One context per host thread (the C ABI's threading rule); the tile and
+    work-order caches are shared by the contexts of a process."""
+    import threading
+    out, errs = {}, []
+
+    def work(tag, sysd, mode):
+        try:
+            with scp3_amd.Context(0) as c:
+                c.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
+                c.set_basis(sysd["modes"], sysd["nmodes2"], sysd["nmodes3"])
+                c.set_pes("qtip4pf")
+                c.set_precision(mode)
+                c.set_tile(4096)
+                for rep in range(3):
+                    c.reset()
+                    c.accumulate(1 << 30, 12288)
+                out[tag] = c.h_download(raw=True)
+        except Exception as e:  # reported below
+            errs.append(repr(e))
+
+    jobs = [("d64", dimer, scp3_amd.FP64), ("hmx", hexamer, scp3_amd.MIXED), ("h64", hexamer, scp3_amd.FP64)]
+    th = [threading.Thread(target=work, args=j) for j in jobs]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    for tag, sysd, mode in jobs:
+        work(tag + "_alone", sysd, mode)
+        np.testing.assert_array_equal(out[tag], out[tag + "_alone"])
