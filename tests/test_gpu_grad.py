@@ -286,3 +286,23 @@ def test_accumulate_gradients_matches_device_potential(ctx, dimer, est):
     with pytest.raises(scp3_amd.Scp3Error, match="gradient estimator"):
         ctx.set_estimator(scp3_amd.EST_ENERGY)
         ctx.accumulate_gradients(1 << 30, E[:10], G[:10])
+
+
+def test_stein_mixed_matches_fp64_20825_states(ctx):
+    """The full hexamer TTM3-F basis (20825 states): mixed against FP64 Stein
+    form on the same 2^15 points, lowest ten levels within 0.5 cm^-1."""
+    from scp3_amd import system
+    print("stein 20825 states mixed vs fp64", flush=True)
+    sysd = system.prepare(system.input_path("hexamer_ttm3f"), nmodes2=48, nmodes3=48)
+    P = 1 << 15
+    ev = {}
+    for mode in (scp3_amd.FP64, scp3_amd.MIXED):
+        setup(ctx, sysd, scp3_amd.EST_GRADIENT_STEIN, tile=0)
+        ctx.set_pes("ttm3f")
+        ctx.set_precision(mode)
+        ctx.accumulate(1 << 30, P)
+        ev[mode] = ctx.lowest_levels(10, tol=1e-7)
+    dlev = np.abs(ev[scp3_amd.MIXED] - ev[scp3_amd.FP64]).max() * scp3_amd.AUTOCM
+    print("stein 20825: max|dlevel| %.4f cm^-1" % dlev, flush=True)
+    assert dlev < 0.5
+    ctx.set_precision(scp3_amd.FP64)
