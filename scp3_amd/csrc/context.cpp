@@ -532,8 +532,10 @@ int scp3g_reset(scp3g_ctx *ctx)
 }  // extern "C"
 
 // The gradient estimator's tile (SCP3.cpp:205-231 with use_gradient):
-// map, q-TIP4P/F energy and gradient, bV, Vq, the two FP64 SYRKs of
-// phi +- s Psi, and the block corrections and fixed elements.
+// map, energy and gradient of the device potential (or the caller's, given
+// = true), bV, Vq, the SYRK pair of phi +- s Psi (two FP64 launches, or one
+// split-BF16 launch over both halves in the mixed form), and the block
+// corrections and fixed elements.
 static int run_tile_grad(scp3g_ctx *ctx, int64_t np, bool given)
 {
     const int64_t T = ctx->work_tile;

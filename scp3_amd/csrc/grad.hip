@@ -20,12 +20,16 @@
 // (e_1 = 1/sqrt2, e_2 = x/sqrt2, e_3 = (2x^2-3)/(2sqrt3)) and
 // w_d = V (4 q_d^2 + 2) - 2 q_d V_d.  On the GPU:
 //
-//   * Psi phi^T + phi Psi^T as two FP64 SYRKs of the LDS-tiled kernel:
-//     (phi + s Psi) with weight 1/(2s) and (phi - s Psi) with weight -1/(2s),
-//     s per point = 1/max(|V|, |V_d|) so both terms are O(phi);
+//   * Psi phi^T + phi Psi^T as a SYRK pair: (phi + s Psi) with weight 1/(2s)
+//     and (phi - s Psi) with weight -1/(2s), s per point = 1/max(|V|, |V_d|)
+//     so both terms are O(phi); in FP64 two launches of the LDS-tiled FP64
+//     SYRK on the tables of k_grad_table, in the mixed form one signed
+//     split-BF16 SYRK over both halves (k_grad_split, syrk_fast.hip);
 //   * the block corrections: for each mode d, the |A_d| x |A_d| upper
 //     triangle over the basis functions A_d that excite d (k_grad_blocks,
-//     FP64 VALU on 64 x 64 or 128 x 128 tiles);
+//     FP64 VALU on 64 x 64 or 128 x 128 tiles; in the mixed form with
+//     blocks over 64 functions, one signed split-BF16 SYRK per mode,
+//     k_corr_split, added by k_corr_scatter);
 //   * M(0,0) = sum V (ScaledMatrixElements.cpp:717) and the reference's own
 //     element placements (below) as a short list of per-point sums
 //     (k_grad_fix);
