@@ -99,7 +99,7 @@ def build_cli(verbose=False):
         os.symlink(real, link)
         extra = ["-DSCP3G_HAVE_HDF5", "-I" + HDF5_INC, real, "-Wl,-rpath,$ORIGIN/hdf5"]
     for src, exe in (("scp3_main.cpp", "SCP3"), ("eigensolver_main.cpp", "eigensolver")):
-        cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", os.path.join(CSRC, src), "-o", os.path.join(BIN, exe),
+        cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-pthread", os.path.join(CSRC, src), "-o", os.path.join(BIN, exe),
                "-L" + HERE, "-lscp3g", "-Wl,-rpath,$ORIGIN/.."] + extra
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <iomanip>
+#include <thread>
 #include <vector>
 
 #include "../../include/PES.h"
@@ -200,6 +201,19 @@ int main(int argc, char **argv)
         if (!rng.good()) die("cannot open " + o.rng_file);
     }
     std::vector<double> Mout;
+#ifdef SCP3G_HAVE_HDF5
+    // sM files are written by a background thread (FP32 conversion and
+    // deflate of up to 1.7 GB per file) while the device keeps accumulating;
+    // one file at a time, joined before the next and at the end
+    std::thread writer;
+    std::vector<double> wbuf;
+    std::string werr;
+    bool wfail = false;
+    auto join_writer = [&]() {
+        if (writer.joinable()) writer.join();
+        if (wfail) die(werr);
+    };
+#endif
     // blocks of bw points as SCP3.cpp:187; consecutive blocks without a
     // checkpoint are sent to the device together
     const int64_t log_every = (int64_t)1 << 14, save_every = (int64_t)1 << 17;
@@ -250,11 +264,23 @@ int main(int argc, char **argv)
                 check(ctx, scp3g_h_download(ctx, Mout.data(), 0), "download");
                 char s[128];
                 snprintf(s, sizeof(s), "sM_%07lld.h5", (long long)i);
-                if (!cli::save_hdf5(s, Mout.data(), n, err)) die(err);
+                join_writer();
+                wbuf.swap(Mout);
+                const std::string fname(s);
+                writer = std::thread([&wbuf, &werr, &wfail, fname, n]() {
+                    std::string e;
+                    if (!cli::save_hdf5(fname, wbuf.data(), n, e)) {
+                        werr = e;
+                        wfail = true;
+                    }
+                });
 #endif
             }
         }
     }
+#ifdef SCP3G_HAVE_HDF5
+    join_writer();
+#endif
     if (o.levels > 0) {
         const int k = (int)std::min<int64_t>(o.levels, n);
         std::vector<double> ev(k);
