@@ -165,7 +165,10 @@ int scp3g_points_done(const scp3g_ctx *ctx, int64_t *npts);
  * (SCP3.cpp:237-239,269-271; ScaledMatrixElements.cpp:735-784), written to
  * M_host[n*n] column-major with the upper triangle filled and the strict
  * lower triangle zero - the layout of the reference's M and of the sM_*.h5
- * dataset.  raw != 0 returns the plain running sum M instead.
+ * dataset.  raw != 0 returns the plain running sum M instead.  For the
+ * gradient estimators M is the sum of the per-point element estimators
+ * (ScaledMatrixElements.cpp:100-725), assembled from the device's running
+ * sums when read.
  */
 int scp3g_h_download(scp3g_ctx *ctx, double *M_host, int raw);
 
