@@ -306,3 +306,13 @@ def test_stein_mixed_matches_fp64_20825_states(ctx):
     print("stein 20825: max|dlevel| %.4f cm^-1" % dlev, flush=True)
     assert dlev < 0.5
     ctx.set_precision(scp3_amd.FP64)
+
+
+@pytest.mark.parametrize("est", [scp3_amd.EST_GRADIENT, scp3_amd.EST_GRADIENT_STEIN])
+def test_grad_sums_mode_subset(ctx, est):
+    """A selection of modes (-m): the estimator over the selected modes with
+    bV over all of them (SCP3.cpp:219-231)."""
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path("dimer_qtip4pf"), nmodes2=5, nmodes3=3,
+                          modes=[1, 3, 4, 6, 7, 9, 11])
+    _check_sums(ctx, sysd, est, 777, 2500)
