@@ -48,6 +48,7 @@ _SIGS = {
     "scp3g_set_mixed_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_cv_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_estimator": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "scp3g_set_scramble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
     "scp3g_pes_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                           ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_int64]),
@@ -202,6 +203,10 @@ class Context:
     def set_estimator(self, est):
         """EST_ENERGY (-G), EST_GRADIENT or EST_GRADIENT_STEIN; clears the running sum."""
         self.check(self.L.scp3g_set_estimator(self._h, int(est)))
+
+    def set_scramble(self, seed):
+        """Affine digit scrambling of the Sobol points (seed != 0), 0 = reference sequence."""
+        self.check(self.L.scp3g_set_scramble(self._h, int(seed)))
 
     def set_mixed_kernel(self, kernel):
         """0: 64 x 64 direct-load SYRKs; 1, 2: LDS-tiled SYRKs (see scp3g.h)"""

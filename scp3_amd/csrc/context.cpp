@@ -361,8 +361,9 @@ int scp3g_set_system(scp3g_ctx *ctx, int nw, const double *x0, int nmodes0,
     ctx->x0.assign(x0, x0 + n3);
     ctx->omega.assign(omega, omega + nmodes0);
     ctx->MUa.assign(MUa, MUa + (size_t)n3 * nmodes0);
-    std::vector<uint64_t> dirs((size_t)nmodes0 * kSobolCols);
+    std::vector<uint64_t> dirs((size_t)nmodes0 * kSobolCols + nmodes0, 0);
     sobol_directions(nmodes0, dirs.data());
+    if (ctx->scramble) sobol_scramble(nmodes0, ctx->scramble, dirs.data(), dirs.data() + (size_t)nmodes0 * kSobolCols);
     dfree(ctx->d_sobol);
     dfree(ctx->d_x0);
     dfree(ctx->d_omega);
@@ -495,6 +496,23 @@ int scp3g_set_estimator(scp3g_ctx *ctx, int estimator)
         grad_release(ctx);
     }
     return 0;
+}
+
+int scp3g_set_scramble(scp3g_ctx *ctx, uint64_t seed)
+{
+    if (!ctx) return -1;
+    if (ctx->nmodes0 <= 0) return fail(ctx, -2, "system not set (scp3g_set_system)");
+    hipSetDevice(ctx->device);
+    const int M = ctx->nmodes0;
+    std::vector<uint64_t> dirs((size_t)M * kSobolCols + M, 0);
+    sobol_directions(M, dirs.data());
+    if (seed) sobol_scramble(M, seed, dirs.data(), dirs.data() + (size_t)M * kSobolCols);
+    int rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (!rc)
+        rc = check_hip(ctx, hipMemcpy(ctx->d_sobol, dirs.data(), dirs.size() * 8, hipMemcpyHostToDevice),
+                       "upload scrambled directions");
+    if (!rc) ctx->scramble = seed;
+    return rc;
 }
 
 int scp3g_set_tile(scp3g_ctx *ctx, int tile)

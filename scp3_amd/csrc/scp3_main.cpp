@@ -18,7 +18,9 @@
 // per thread, as the reference does; sampling, basis and SYRK stay on the
 // GPU: the path for potentials without a device kernel, such as WHBB; with
 // the PES's gradient overload without -G), --stein (the gradient estimator with every element from the Stein
-// identity instead of the reference's placement of five element families).
+// identity instead of the reference's placement of five element families),
+// --scramble SEED (affine digit scrambling of the Sobol points, in place of
+// the reference's external scrambled -r files).
 // Files: omega0.dat, E0_sd.dat (every 2^14 points), E0_t.dat (empty),
 // sM_%07d.h5 (every 2^17 points), levels.dat (this driver's final levels).
 #include <getopt.h>
@@ -43,6 +45,7 @@ struct Opts {
     std::string rng_file, pes = "qtip4pf", input, modes_desc;
     int n2 = 0, n3 = 0, bw = 1024, levels = 10, device = 0;
     bool gradient = true, have_modes = false, cpu_pes = false, stein = false;
+    unsigned long long scramble = 0;
     int precision = SCP3G_FP64;
 };
 
@@ -87,6 +90,7 @@ Opts parse(int argc, char **argv)
                           {"device", required_argument, nullptr, 1002},
                           {"cpu-pes", no_argument, nullptr, 1003},
                           {"stein", no_argument, nullptr, 1004},
+                          {"scramble", required_argument, nullptr, 1005},
                           {nullptr, 0, nullptr, 0}};
     Opts o;
     int ch;
@@ -123,6 +127,7 @@ Opts parse(int argc, char **argv)
         case 1002: o.device = atoi(optarg); break;
         case 1003: o.cpu_pes = true; break;
         case 1004: o.stein = true; break;
+        case 1005: o.scramble = strtoull(optarg, nullptr, 0); break;
         default: die("Unknown option: " + std::to_string(ch));
         }
     }
@@ -179,6 +184,7 @@ int main(int argc, char **argv)
     if (scp3g_create(&ctx, o.device)) die("no usable GPU");
     check(ctx, scp3g_set_system(ctx, N / 3, in.x0.data(), nm0, omega.data(), MUa.data()), "set_system");
     check(ctx, scp3g_set_basis(ctx, (int)modes.size(), modes.data(), o.n2, o.n3), "set_basis");
+    if (o.scramble) check(ctx, scp3g_set_scramble(ctx, o.scramble), "set_scramble");
     if (!o.cpu_pes) check(ctx, scp3g_set_pes(ctx, o.pes.c_str()), "set_pes");
     check(ctx, scp3g_set_precision(ctx, o.precision), "set_precision");
     if (o.gradient)

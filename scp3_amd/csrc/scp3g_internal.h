@@ -70,9 +70,11 @@ struct scp3g_ctx {
     bool cv_one_term = false;
     // SCP3G_EST_ENERGY (the -G estimator) or one of the gradient forms
     int estimator = SCP3G_EST_ENERGY;
+    // Sobol digit scrambling seed, 0 = the reference's sequence
+    uint64_t scramble = 0;
 
     // device copies of the set-up
-    uint64_t *d_sobol = nullptr;  // nmodes0 x 62
+    uint64_t *d_sobol = nullptr;  // nmodes0 x 62, then nmodes0 shifts (scrambled)
     double *d_x0 = nullptr, *d_omega = nullptr, *d_MUa = nullptr;
     int *d_modes = nullptr;
     scp3g::RowDesc *d_desc = nullptr;
@@ -212,6 +214,8 @@ void release_davidson(scp3g_ctx *ctx);
 // host helpers
 std::vector<double> ho_diagonal(const scp3g_ctx *ctx);  // HO diagonal + unselected ZPE
 void sobol_directions(int dims, uint64_t *v);  // dims x 62, scaled
+// affine digit scrambling of the direction integers in place, shifts dims
+void sobol_scramble(int dims, uint64_t seed, uint64_t *v, uint64_t *shift);
 // lowest nvec eigenpairs of a small dense symmetric matrix (column-major,
 // both triangles, overwritten); w ascending, Z m x nvec (small_eigh.cpp)
 int small_eigh(int m, std::vector<double> &A, int nvec, std::vector<double> &w, std::vector<double> &Z);

@@ -43,6 +43,46 @@ void sobol_directions(int dims, uint64_t *v)
     }
 }
 
+// Affine matrix scrambling of the Sobol digits (Matousek 1998; the class of
+// MATLAB's scramble(p, 'MatousekAffineOwen') behind the reference's -r
+// files): per dimension a random lower-triangular binary matrix L with unit
+// diagonal in digit order (output digit i depends on input digits 0..i,
+// digit 0 the most significant, bit 61) and a random digital shift e, so a
+// point x becomes L x + e over GF(2).  L is linear, so it is applied once to
+// the direction integers (L of the XOR is the XOR of L) and e starts each
+// lane's state.  Random bits: splitmix64 seeded with seed ^ (0x9e3779b97f4a7c15
+// * (d + 1)), one draw per digit (the bits of L's row above the diagonal),
+// then one for e.
+static uint64_t splitmix64(uint64_t &st)
+{
+    uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+void sobol_scramble(int dims, uint64_t seed, uint64_t *v, uint64_t *shift)
+{
+    const uint64_t all = (1ull << kSobolCols) - 1;
+    for (int d = 0; d < dims; d++) {
+        uint64_t st = seed ^ (0x9e3779b97f4a7c15ull * (uint64_t)(d + 1));
+        uint64_t mask[kSobolCols];
+        for (int i = 0; i < kSobolCols; i++) {
+            const int b = kSobolCols - 1 - i;
+            const uint64_t above = all & ~((2ull << b) - 1);
+            mask[i] = (1ull << b) | (splitmix64(st) & above);
+        }
+        shift[d] = splitmix64(st) & all;
+        uint64_t *row = v + (size_t)d * kSobolCols;
+        for (int j = 0; j < kSobolCols; j++) {
+            uint64_t out = 0;
+            for (int i = 0; i < kSobolCols; i++)
+                out |= (uint64_t)(__builtin_popcountll(row[j] & mask[i]) & 1) << (kSobolCols - 1 - i);
+            row[j] = out;
+        }
+    }
+}
+
 // beasley_springer_moro.cpp:18-23
 __constant__ double kBsmA[4] = {2.50662823884, -18.61500062529, 41.39119773534,
                                 -25.44106049637};
@@ -81,8 +121,8 @@ constexpr int kSobolPPT = 16;  // points per lane (strided by 64)
 
 // grid: x = groups of (waves per block * 64 * PPT) points, y = dimension
 __global__ void __launch_bounds__(256)
-    k_sobol(const uint64_t *__restrict__ dirs, int64_t skip, int64_t npts, int halve,
-            double *__restrict__ y, int64_t ldy, uint64_t *__restrict__ ints)
+    k_sobol(const uint64_t *__restrict__ dirs, const uint64_t *__restrict__ shift, int64_t skip,
+            int64_t npts, int halve, double *__restrict__ y, int64_t ldy, uint64_t *__restrict__ ints)
 {
 #pragma clang fp contract(off)
     const int d = blockIdx.y;
@@ -94,7 +134,7 @@ __global__ void __launch_bounds__(256)
     if (i0 >= npts) return;
     uint64_t s = (uint64_t)(skip + i0);
     uint64_t g = s ^ (s >> 1);
-    uint64_t st = 0;
+    uint64_t st = shift ? shift[d] : 0;
     while (g) {
         st ^= v[__builtin_ctzll(g)];
         g &= g - 1;
@@ -123,7 +163,10 @@ hipError_t launch_sobol(scp3g_ctx *ctx, const uint64_t *dirs, int dims, int64_t 
     const int waves = 4;
     const int64_t per_block = (int64_t)waves * 64 * kSobolPPT;
     dim3 grid((unsigned)((npts + per_block - 1) / per_block), (unsigned)dims);
-    hipLaunchKernelGGL(k_sobol, grid, dim3(64 * waves), 0, ctx->stream, dirs, skip, npts,
+    // the context's own table carries the digital shifts of a scrambled
+    // sequence behind its direction integers (scp3g_set_scramble)
+    const uint64_t *shift = (dirs == ctx->d_sobol && ctx->scramble) ? dirs + (size_t)dims * kSobolCols : nullptr;
+    hipLaunchKernelGGL(k_sobol, grid, dim3(64 * waves), 0, ctx->stream, dirs, shift, skip, npts,
                        halve, y, ldy, ints);
     return hipGetLastError();
 }

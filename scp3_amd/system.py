@@ -61,6 +61,22 @@ def write_vladimir(path, species, x, H, comment="Free energy = 0"):
                 f.write("%.17e\n" % H[j, i])
 
 
+def write_normals(path, normals):
+    """A -r rng file (SCP3.cpp:188-194): one line per point with its nmodes0
+    standard normals (not yet divided by sqrt(2)), whitespace-separated."""
+    Z = np.atleast_2d(np.asarray(normals, dtype=np.float64))
+    with open(path, "w") as f:
+        for row in Z:
+            f.write(" ".join("%.17g" % v for v in row) + "\n")
+
+
+def read_normals(path, nmodes0):
+    """The normals of a -r rng file, points x nmodes0 (the reference reads
+    them in this order with operator>>)."""
+    v = np.loadtxt(path, dtype=np.float64).ravel()
+    return v[: v.size // nmodes0 * nmodes0].reshape(-1, nmodes0)
+
+
 def ohhohh(species, x, H=None):
     """Permute coordinates (and H) into O H H blocks (DiskIO.cpp:142-195)."""
     n3 = len(x)

@@ -420,3 +420,62 @@ def test_two_contexts_in_two_threads(dimer, hexamer):
     for tag, sysd, mode in jobs:
         work(tag + "_alone", sysd, mode)
         np.testing.assert_array_equal(out[tag], out[tag + "_alone"])
+
+
+M64 = (1 << 64) - 1
+
+
+def _splitmix64(st):
+    st = (st + 0x9e3779b97f4a7c15) & M64
+    z = st
+    z = ((z ^ (z >> 30)) * 0xbf58476d1ce4e5b9) & M64
+    z = ((z ^ (z >> 27)) * 0x94d049bb133111eb) & M64
+    return st, z ^ (z >> 31)
+
+
+def _scramble_ints(x, seed):
+    """Affine digit scramble L x + e per dimension (the library's documented
+    generator), restated on the oracle's Sobol integers (points x dims)."""
+    out = np.zeros_like(x)
+    allb = (1 << 62) - 1
+    for d in range(x.shape[1]):
+        st = (seed ^ ((0x9e3779b97f4a7c15 * (d + 1)) & M64)) & M64
+        masks = []
+        for i in range(62):
+            b = 61 - i
+            st, r = _splitmix64(st)
+            masks.append((1 << b) | (r & allb & ~((2 << b) - 1)))
+        st, e = _splitmix64(st)
+        e &= allb
+        col = []
+        for v in x[:, d].tolist():
+            o = 0
+            for i, m in enumerate(masks):
+                o |= (bin(v & m).count("1") & 1) << (61 - i)
+            col.append(o ^ e)
+        out[:, d] = np.array(col, dtype=np.uint64)
+    return out
+
+
+def test_scrambled_points(ctx, dimer):
+    """scp3g_set_scramble: the points are L x + e of the reference's Sobol
+    integers (checked through the mapped Cartesians), differ between seeds,
+    and seed 0 restores the reference sequence."""
+    setup_system(ctx, dimer, tile=1024)
+    P, skip, seed = 700, 1 << 30, 0x5eed
+    M0 = dimer["omega"].size
+    ctx.set_scramble(seed)
+    X = ctx.points_coords(skip, P)
+    ints = _scramble_ints(oracle.sobol_ints(M0, skip, P), seed)
+    u = ints.astype(np.float64) * 2.0 ** -62
+    y = oracle.bsm(u.ravel()).reshape(u.shape) / np.sqrt(2.0)
+    ref = scp3_amd.BOHR * (dimer["x0"][None, :] + y @ dimer["MUa"].T)
+    assert np.abs(X - ref).max() < 1e-11 * np.abs(ref).max()
+    ctx.set_scramble(seed + 1)
+    X2 = ctx.points_coords(skip, P)
+    assert np.abs(X2 - X).max() > 1e-3
+    ctx.set_scramble(0)
+    X0 = ctx.points_coords(skip, P)
+    y0 = oracle.sobol_normals(M0, skip, P)
+    ref0 = scp3_amd.BOHR * (dimer["x0"][None, :] + y0 @ dimer["MUa"].T)
+    assert np.abs(X0 - ref0).max() < 1e-11 * np.abs(ref0).max()

@@ -57,3 +57,12 @@ def test_inputs_are_minima():
         _, xs, _, _ = system.ohhohh(sp, x)
         E, g = scp3_amd.pes_cpu("qtip4pf", xs, grad=True)
         assert np.abs(g).max() < 1e-4
+
+
+def test_normals_file_round_trip(tmp_path):
+    from scp3_amd import system
+    rng = np.random.default_rng(4)
+    Z = rng.normal(size=(37, 12))
+    path = str(tmp_path / "normals.txt")
+    system.write_normals(path, Z)
+    np.testing.assert_array_equal(system.read_normals(path, 12), Z)
