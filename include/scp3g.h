@@ -281,6 +281,32 @@ int scp3g_pes_gradient(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const doub
                        int64_t ldc, double *E_dev, double *G_dev, int64_t ldg);
 
 /*
+ * SCP1's Gaussian averages (SCP1::average, SCP1.cpp:33-86) over the Sobol
+ * points skip .. skip + npts - 1 in 9nw dimensions, on this device (the
+ * reference splits the range over MPI ranks and Allreduces the three sums).
+ * Point i: z = std_normal(9nw, i), dq = isqrtM_U (z o sqrt(d)), r = bohr (q +
+ * dq) in Angstrom, V and g = grad V of potential pes (SCP3G_PES_*) at r.
+ * Host arrays: q (9nw, bohr, O H H order), d (9nw, >= 0), isqrtM_U (9nw x
+ * 9nw column-major).  Outputs (host): *Upot = <V>/autokcalpmol,
+ * UX = <g> bohr/autokcalpmol (9nw), UX_q = <(g bohr/autokcalpmol) dq^T>
+ * (9nw x 9nw column-major), each divided by npts.  Up to 21 waters (the
+ * potential kernels' limit).
+ */
+int scp3g_scp1_average(scp3g_ctx *ctx, int pes, int nw, const double *q, const double *d,
+                       const double *isqrtM_U, int64_t skip, int64_t npts, double *Upot,
+                       double *UX, double *UX_q);
+
+/*
+ * Cartesian Hessian of potential pes at q (9nw, bohr) by central differences
+ * of the gradient with step s bohr, all 2 x 9nw displaced configurations in
+ * one batch (getHessian, Hessian.cpp:13-43).  H (host, 9nw x 9nw
+ * column-major) in Hartree/bohr^2: the gradient is converted to Hartree/bohr
+ * before the difference (the reference divides kcal/mol/A by the step in
+ * bohr).  As in the reference only the upper triangle is symmetrised.
+ */
+int scp3g_hessian(scp3g_ctx *ctx, int pes, int nw, const double *q, double s, double *H);
+
+/*
  * Basis table B_dev[a*ldb + i] = bra_a(q_i) for the context's basis, with
  * q_i[m] = y_dev[modes[m]*ldy + i] (get_bra, ScaledMatrixElements.cpp:870-930).
  * fp32 != 0 writes float.

@@ -64,6 +64,7 @@ def lib():
         L.orc_ttm3f_grad.argtypes = [I64, P, P, D, P]
         L.orc_ttm3f_grad.restype = D
         L.orc_scp3_accumulate_grad.argtypes = [I, I, P, P, P, I, P, I, I, I, I, I64, I64, P, P, P, P]
+        L.orc_scp1_average.argtypes = [I, I, P, P, P, I64, I64, P, P, P]
         if hasattr(L, "orc_ttm3f"):
             try:
                 L.orc_ttm3f.argtypes = [I64, P]
@@ -252,6 +253,24 @@ def scp3_accumulate_grad(sysd, seed0, npts, form=0, normals=None, pes="qtip4pf")
                                     _p(Mm), _p(bV), _p(Vq))
     assert rc == 0, rc
     return dict(M=Mm, bV=bV, Vq=Vq)
+
+
+def scp1_average(pes, q, d, U, skip, npts):
+    """SCP1::average (SCP1.cpp:33-86): <V>, <UX>, <UX q^T> in Hartree / bohr
+    over the Sobol points skip .. skip + npts - 1 (scp1_oracle.c)."""
+    q = np.ascontiguousarray(q, dtype=np.float64).ravel()
+    d = np.ascontiguousarray(d, dtype=np.float64).ravel()
+    Uf = np.asfortranarray(U, dtype=np.float64)
+    n3 = q.size
+    UX = np.zeros(n3)
+    UXq = np.zeros((n3, n3), order="F")
+    Up = ctypes.c_double()
+    pid = {"qtip4pf": 0, "ttm3f": 1}[pes]
+    rc = lib().orc_scp1_average(n3 // 9, pid, _p(q), _p(d), ctypes.c_void_p(Uf.ctypes.data), int(skip),
+                                int(npts), ctypes.byref(Up), _p(UX), ctypes.c_void_p(UXq.ctypes.data))
+    if rc:
+        raise RuntimeError("orc_scp1_average: %d" % rc)
+    return Up.value, UX, UXq
 
 
 def finalize(sysd, Mraw, npts):

@@ -49,6 +49,11 @@ _SIGS = {
     "scp3g_set_cv_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_estimator": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_scramble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    "scp3g_hessian": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_double, ctypes.c_void_p]),
+    "scp3g_scp1_average": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "scp3g_pes_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                           ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_int64]),
@@ -315,6 +320,35 @@ class Context:
         E = np.ascontiguousarray(E, dtype=np.float64)
         G = np.ascontiguousarray(G, dtype=np.float64)
         self.check(self.L.scp3g_accumulate_gradients(self._h, int(skip), E.size, _ptr(E), _ptr(G)))
+
+    def scp1_average(self, pes, q, d, isqrtM_U, skip, npts):
+        """SCP1's averages <V>, <UX>, <UX q^T> (Hartree, bohr) over the Sobol
+        points skip .. skip + npts - 1; q, d (9nw), isqrtM_U (9nw x 9nw)."""
+        q = np.ascontiguousarray(q, dtype=np.float64).ravel()
+        d = np.ascontiguousarray(d, dtype=np.float64).ravel()
+        n3 = q.size
+        Um = np.asfortranarray(isqrtM_U, dtype=np.float64)
+        if d.size != n3 or Um.shape != (n3, n3) or n3 % 9:
+            raise ValueError("q, d: 9nw values; isqrtM_U: 9nw x 9nw")
+        pid = {"qtip4pf": PES_QTIP4PF, "ttm3f": PES_TTM3F}[pes] if isinstance(pes, str) else int(pes)
+        U = ctypes.c_double()
+        UX = np.zeros(n3)
+        UXq = np.zeros((n3, n3), order="F")
+        self.check(self.L.scp3g_scp1_average(self._h, pid, n3 // 9, _ptr(q), _ptr(d), ctypes.c_void_p(Um.ctypes.data),
+                                             int(skip), int(npts), ctypes.byref(U), _ptr(UX),
+                                             ctypes.c_void_p(UXq.ctypes.data)))
+        return U.value, UX, UXq
+
+    def hessian(self, pes, q, s=1e-3):
+        """Cartesian Hessian (Hartree/bohr^2) at q (bohr) by central
+        differences of the gradient, step s bohr (getHessian)."""
+        q = np.ascontiguousarray(q, dtype=np.float64).ravel()
+        if q.size % 9:
+            raise ValueError("q: 9nw values")
+        pid = {"qtip4pf": PES_QTIP4PF, "ttm3f": PES_TTM3F}[pes] if isinstance(pes, str) else int(pes)
+        H = np.zeros((q.size, q.size), order="F")
+        self.check(self.L.scp3g_hessian(self._h, pid, q.size // 9, _ptr(q), float(s), ctypes.c_void_p(H.ctypes.data)))
+        return H
 
     def accumulate_surrogate(self, skip, npts):
         self.check(self.L.scp3g_accumulate_surrogate(self._h, int(skip), int(npts)))

@@ -27,6 +27,7 @@ SOURCES = [
     "eigen.hip",
     "mlp.hip",
     "grad.hip",
+    "scp1.hip",
     "context.cpp",
     "modes.cpp",
     "pes_cpu.cpp",
@@ -80,8 +81,8 @@ BIN = os.path.join(HERE, "bin")
 
 
 def build_cli(verbose=False):
-    """The SCP3 and eigensolver drivers, linked against libscp3g.so (and the
-    HDF5 C library for sM_*.h5 when its header is present)."""
+    """The SCP3, eigensolver and SCP1 drivers, linked against libscp3g.so (and
+    the HDF5 C library for sM_*.h5 when its header is present)."""
     os.makedirs(BIN, exist_ok=True)
     hdf5 = os.path.exists(os.path.join(HDF5_INC, "hdf5.h"))
     extra = []
@@ -98,7 +99,7 @@ def build_cli(verbose=False):
             os.remove(link)
         os.symlink(real, link)
         extra = ["-DSCP3G_HAVE_HDF5", "-I" + HDF5_INC, real, "-Wl,-rpath,$ORIGIN/hdf5"]
-    for src, exe in (("scp3_main.cpp", "SCP3"), ("eigensolver_main.cpp", "eigensolver")):
+    for src, exe in (("scp3_main.cpp", "SCP3"), ("eigensolver_main.cpp", "eigensolver"), ("scp1_main.cpp", "SCP1")):
         cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-pthread", os.path.join(CSRC, src), "-o", os.path.join(BIN, exe),
                "-L" + HERE, "-lscp3g", "-Wl,-rpath,$ORIGIN/.."] + extra
         r = subprocess.run(cmd, capture_output=True, text=True)

@@ -319,6 +319,7 @@ void scp3g_destroy(scp3g_ctx *ctx)
     free_work(ctx);
     free_sums(ctx);
     grad_release(ctx);
+    scp1_release(ctx);
     dfree(ctx->d_part);
     dfree(ctx->d_sobol);
     dfree(ctx->d_x0);

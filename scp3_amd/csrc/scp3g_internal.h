@@ -188,6 +188,8 @@ hipError_t launch_ttm3f_grad(scp3g_ctx *ctx, int nw, int64_t npts, const double 
 // gradient estimator (grad.hip)
 int grad_setup(scp3g_ctx *ctx);
 void grad_release(scp3g_ctx *ctx);
+// SCP1 averages (scp1.hip)
+void scp1_release(scp3g_ctx *ctx);
 hipError_t launch_vq(scp3g_ctx *ctx, const double *G, int64_t ldg, const double *y, int64_t ldy,
                      const double *w, int64_t npts, double *Vq, int64_t ldv, double *wp, double *wm,
                      double *sc);

@@ -132,3 +132,14 @@ def test_cpu_pes_run_matches_device_run(tmp_path):
         assert r.returncode == 0, r.stderr
         out[tag] = np.loadtxt(d / "levels.dat")
     np.testing.assert_allclose(out["cpu"][:, 0], out["dev"][:, 0], rtol=0, atol=1e-10)
+
+
+def test_scp1_options(tmp_path):
+    r = run(["SCP1"], str(tmp_path))
+    assert r.returncode != 0 and "No input file supplied" in r.stderr
+    r = run(["SCP1", "-p", "whbb", DIMER], str(tmp_path))
+    assert r.returncode != 0 and "no GPU implementation" in r.stderr
+    r = run(["SCP1", "-N", "41", DIMER], str(tmp_path))
+    assert r.returncode != 0 and "too large" in r.stderr
+    r = run(["SCP1", str(tmp_path / "missing.dat")], str(tmp_path))
+    assert r.returncode != 0 and "File not found" in r.stderr
