@@ -20,7 +20,9 @@ control variate: 2^24 surrogate + 2^19 true points against plain FP64 at
 2^21), c5 (20-water cluster under TTM3-F, 174 modes, 15400 states, MLP
 174-512-512-512-1 on 2^19 pairs, 2^24 surrogate + 2^20 true points against
 plain FP64 at 2^20, lowest 20 levels); c3g (c3 with the gradient
-estimator, Stein form, 2^20 points, FP64 against mixed).
+estimator, Stein form, 2^20 points, FP64 against mixed); scp1 / scp1q
+(SCP1's Gaussian averages: hexamer under TTM3-F at 2^18 points / under
+q-TIP4P/F at 2^20 points, one step = one SCP1 iteration's averages).
 
 Forms: fp64, mixed, cv (above) and grad / stein: the gradient (Stein-
 identity) estimator, the reference's default without -G (grad: the
@@ -60,6 +62,10 @@ PRESETS = {
                cv_nn=24, cv_t=20, mlp_hidden=512, mlp_steps=20000, mlp_batch=4096, mlp_pairs=19),
     # c3 with the gradient (Stein) estimator: FP64 against the mixed form
     "c3g": dict(config="hexamer_ttm3f", n2=48, n3=48, points=20, tile=0, levels=10, forms="stein,stein_mixed"),
+    # SCP1's Gaussian averages (energy, gradient and the gradient-displacement
+    # moments) over the Sobol points, one SCP1 iteration's worth
+    "scp1": dict(kind="scp1", config="hexamer_ttm3f", pes="ttm3f", points=18),
+    "scp1q": dict(kind="scp1", config="hexamer_qtip4pf", pes="qtip4pf", points=20),
 }
 
 
@@ -225,6 +231,77 @@ def cpu_baseline(sysd, skip, chunk, target_s, max_points):
                        % (done, chunk, total, {k: round(v, 3) for k, v in t.items()}, cores))
 
 
+def scp1_modes(path):
+    """q (bohr), mass, d and isqrtM_U of an input's Hessian at T = 0, as the
+    SCP1 driver starts (SCP1.cpp:105-110; canonical eigenvector signs)."""
+    from scp3_amd import AUTOCM, BOHR, system
+    species, x, H = system.read_vladimir(path)
+    mass, xs, Hs, _ = system.ohhohh(species, x, H)
+    w, U = np.linalg.eigh(Hs, UPLO="U")
+    U = U * np.sign(U[np.argmax(np.abs(U), axis=0), np.arange(U.shape[1])])[None, :]
+    om = np.sqrt(np.abs(w))
+    with np.errstate(divide="ignore"):
+        d = np.where(om < 50.0 / AUTOCM, 0.0, 0.5 / om)
+    return xs / BOHR, mass, d, U / np.sqrt(mass)[:, None]
+
+
+def bench_scp1(a, pr):
+    """One step = scp3g_scp1_average over 2^points Sobol points: normals,
+    map, energy and gradient, moments, as one SCP1 iteration does."""
+    import scp3_amd
+    from scp3_amd import system
+    P = 1 << pr["points"]
+    skip = P * 2  # SCP1.h:56
+    q, mass, d, iU = scp1_modes(system.input_path(pr["config"]))
+    with scp3_amd.Context(0) as c:
+        for _ in range(a.warmup):
+            c.scp1_average(pr["pes"], q, d, iU, skip, P)
+        c.set_timing(True)
+        c.sync()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            U0, UX, UXq = c.scp1_average(pr["pes"], q, d, iU, skip, P)
+        c.sync()
+        t = (time.perf_counter() - t0) / a.steps
+        kern = {k: c.kernel_time(k) for k in ("scp1_sobol", "scp1_map", "scp1_pes", "scp1_moments")}
+    n3 = q.size
+    out = {
+        "metric": "SCP1 Gaussian averages: Sobol points/s (energy, gradient, <UX dq^T>)",
+        "value": P / t,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * t,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seed-built relaxed cluster (scp3_amd/data), Sobol points",
+        "config": {"workload": "%s: %s SCP1 averages, %d Cartesian dims, 2^%d Sobol points (skip 2^%d), "
+                               "131072-point tiles" % (a.preset, pr["config"], n3, pr["points"], pr["points"] + 1),
+                   "points": P},
+        # the potential kernel dominates and is bound by FP64 vector issue,
+        # neither HBM nor MFMA
+        "roofline": None,
+        "kernel_ms_per_step": {k: round(v[0] / a.steps, 4) for k, v in kern.items()},
+        "Upot_hartree": U0,
+    }
+    if not a.no_cpu:
+        from oracle import oracle
+        done, tc, chunk = 0, 0.0, 512
+        while tc < a.cpu_seconds and done < P:
+            t0 = time.perf_counter()
+            oracle.scp1_average(pr["pes"], q, d, iU, skip + done, chunk)
+            tc += time.perf_counter() - t0
+            done += chunk
+        out["cpu_baseline"] = {"value": done / tc, "unit": "points/s", "cores": 1, "kind": "port",
+                               "sample": "%d of the same Sobol points in %d-point chunks (%.1f s): the oracle's "
+                                         "serial restatement of SCP1::average (one MPI rank's loop)"
+                                         % (done, chunk, tc)}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.copy_bytes:
@@ -240,6 +317,9 @@ def main():
     pr = dict(PRESETS[a.preset])
     if a.points is not None:
         pr["points"] = a.points
+    if pr.get("kind") == "scp1":
+        bench_scp1(a, pr)
+        return
     if a.forms:
         pr["forms"] = a.forms
     if a.tile is not None:

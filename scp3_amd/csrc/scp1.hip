@@ -230,12 +230,15 @@ int scp3g_scp1_average(scp3g_ctx *ctx, int pes, int nw, const double *q, const d
         (rc = check_hip(ctx, hipMemsetAsync(acc, 0, mm * 8, ctx->stream), "clear scp1 sums")))
         return rc;
 
-    hipEvent_t ev;
+    hipEvent_t ev, ek;
     timer_begin(ctx, "scp1", &ev);
     for (int64_t done = 0; done < npts && !rc; done += T) {
         const int64_t np = std::min<int64_t>(T, npts - done);
+        timer_begin(ctx, "scp1_sobol", &ek);
         rc = check_hip(ctx, launch_sobol(ctx, w.dirs, n3, skip + done, np, 0, zG, ld, nullptr), "scp1 sobol");
+        timer_end(ctx, "scp1_sobol", ek);
         if (rc) break;
+        timer_begin(ctx, "scp1_map", &ek);
         dim3 mgrid((unsigned)((np + 255) / 256), (unsigned)((n3 + nc - 1) / nc));
         const size_t smem = sizeof(double) * n3 * nc;
         if (nc == 18)
@@ -244,16 +247,21 @@ int scp3g_scp1_average(scp3g_ctx *ctx, int pes, int nw, const double *q, const d
             hipLaunchKernelGGL(k_scp1_map<54>, mgrid, dim3(256), smem, ctx->stream, zG, ld, n3, sd, Ud, dq0, np, dq, r);
         else
             hipLaunchKernelGGL(k_scp1_map<30>, mgrid, dim3(256), smem, ctx->stream, zG, ld, n3, sd, Ud, dq0, np, dq, r);
+        timer_end(ctx, "scp1_map", ek);
         if ((rc = check_hip(ctx, hipGetLastError(), "scp1 map"))) break;
         // the normals are dead once mapped: their buffer takes the gradients
+        timer_begin(ctx, "scp1_pes", &ek);
         rc = pes == SCP3G_PES_TTM3F
                  ? check_hip(ctx, launch_ttm3f_grad(ctx, nw, np, r, ld, E, zG, ld), "scp1 ttm3f")
                  : check_hip(ctx, launch_qtip4pf_grad(ctx, nw, np, r, ld, E, zG, ld), "scp1 qtip4pf");
+        timer_end(ctx, "scp1_pes", ek);
         if (rc) break;
         const int ch = partial_chunks(np);
         const unsigned tiles = (unsigned)((m + kMt - 1) / kMt);
+        timer_begin(ctx, "scp1_moments", &ek);
         hipLaunchKernelGGL(k_scp1_moments, dim3(tiles, tiles, (unsigned)ch), dim3(256), 0, ctx->stream, zG, E, dq,
                            ld, n3, np, part);
+        timer_end(ctx, "scp1_moments", ek);
         if ((rc = check_hip(ctx, hipGetLastError(), "scp1 moments"))) break;
         hipLaunchKernelGGL(k_scp1_fold, dim3((unsigned)((mm + 255) / 256)), dim3(256), 0, ctx->stream, part, ch,
                            (int64_t)mm, acc);

@@ -147,3 +147,28 @@ def test_scp1_driver_ttm3f(tmp_path):
     assert r.returncode == 0, r.stderr
     q0, mass, Hs = load("hexamer_ttm3f")
     check_against_oracle(tmp_path, q0, mass, Hs, "ttm3f", 10, 2, tol=100.0)
+
+
+def test_scp1_reproduces_reference_prism_free_energy(tmp_path):
+    """tests/golden/prism_T0K_1M_D50.dat is the reference's own SCP1 result
+    for the water hexamer prism (examples/prism_T0K_1M_D50.dat: q-TIP4P/F,
+    T = 0 K, 2^20 points, 50 cm^-1 cutoff), whose header holds the free
+    energy it reached, 45.5909740976476 kcal/mol.  Twenty iterations from
+    its geometry and effective Hessian stay at that fixed point: with
+    qeps = 2 the iteration alternates between two states about 0.006
+    kcal/mol apart, and the upper one matches the reference's value."""
+    print("scp1 driver: reference prism, q-TIP4P/F, -N 20 -L 20", flush=True)
+    path = os.path.join(ROOT, "tests", "golden", "prism_T0K_1M_D50.dat")
+    with open(path) as f:
+        f.readline()
+        F_ref = float(f.readline().split("=")[1].split()[0])
+    r = run_scp1(["-N", "20", "-L", "20", path], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    F = np.loadtxt(tmp_path / "free_energy.dat") * S.AUTOKCALPMOL
+    assert F.size == 20
+    assert np.all(np.abs(F - F_ref) < 0.01)
+    assert abs(F[-10:].max() - F_ref) < 0.002
+    species, x, _ = system.read_vladimir(path)
+    _, xs, _, _ = system.ohhohh(species, x, None)
+    q, _ = read_coord(str(tmp_path / "coord.xyz"), xs.size)
+    assert np.abs(q * S.BOHR - xs).max() < 1e-3
