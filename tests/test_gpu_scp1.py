@@ -42,6 +42,7 @@ def ctx():
     ("dimer_qtip4pf", "qtip4pf", (1 << 17) + 1000),   # two tiles, ragged end
     ("hexamer_qtip4pf", "qtip4pf", 5000),
     ("hexamer_ttm3f", "ttm3f", 1500),
+    ("w20_ttm3f", "ttm3f", 300),                      # 180 dims: the map in 30-row chunks
 ])
 def test_average_matches_oracle(ctx, name, pes, npts):
     print("scp1 average:", name, pes, npts, flush=True)
