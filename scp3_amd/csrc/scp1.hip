@@ -184,8 +184,15 @@ int scp3g_scp1_average(scp3g_ctx *ctx, int pes, int nw, const double *q, const d
     const size_t mm = (size_t)m * m;
     const size_t nbuf = 3 * (size_t)n3 * T + T + (size_t)nch * mm + mm;
 
-    std::lock_guard<std::mutex> guard(g_scp1_lock);
-    Scp1Work &w = g_scp1[ctx];
+    // the lock guards the map; a context's own entry is used by that
+    // context's calls only (a context is not shared between threads), and
+    // map nodes stay in place while other contexts add theirs
+    Scp1Work *wp;
+    {
+        std::lock_guard<std::mutex> guard(g_scp1_lock);
+        wp = &g_scp1[ctx];
+    }
+    Scp1Work &w = *wp;
     int rc = 0;
     if (w.dev != ctx->device || w.n3 != n3 || w.T < T) {
         hipFree(w.buf);
@@ -200,7 +207,7 @@ int scp3g_scp1_average(scp3g_ctx *ctx, int pes, int nw, const double *q, const d
             hipFree(w.buf);
             hipFree(w.dirs);
             hipFree(w.aux);
-            g_scp1.erase(ctx);
+            w = Scp1Work();
             return rc;
         }
         if ((rc = check_hip(ctx, hipMemcpy(w.dirs, dirs.data(), dirs.size() * 8, hipMemcpyHostToDevice),
