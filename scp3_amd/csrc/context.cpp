@@ -321,6 +321,7 @@ void scp3g_destroy(scp3g_ctx *ctx)
     grad_release(ctx);
     scp1_release(ctx);
     dfree(ctx->d_part);
+    dfree(ctx->d_mono);
     dfree(ctx->d_sobol);
     dfree(ctx->d_x0);
     dfree(ctx->d_omega);

@@ -816,13 +816,47 @@ __device__ void pair_dipole_grad(const double *x, const double *ms, const double
     }
 }
 
+// The monomer terms of the gradient, one lane per (configuration,
+// molecule) so that every lane works (inside k_ttm3f_grad only nw of 64
+// would): PS energy and gradient (ps.cpp:486-603) and DMS charges with
+// their derivatives (ps.cpp:605-836, scaled as ttm3f.cpp:357-392).
+// mono[f * ldm + cfg * nw + w]: f = 0 energy, 1-9 PS gradient, 10-12
+// charges, 13-39 charge derivatives.
+constexpr int kMonoFields = 40;
+
+__global__ void __launch_bounds__(256)
+    k_ttm3f_monomer_grad(const double *__restrict__ crd, int64_t ldc, int nw, int64_t npts,
+                         double *__restrict__ mono, int64_t ldm)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= npts * nw) return;
+    const int64_t cfg = idx / nw;
+    const int w = (int)(idx - cfg * nw);
+    double x[9], g[9], q[3], gq[27];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) x[k] = crd[(size_t)(9 * w + k) * ldc + cfg];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) gq[k] = 0.0;
+    const double e = ps_energy_grad(x, x + 3, x + 6, g);
+    dms_charges_grad(x, x + 3, x + 6, q, gq);
+    mono[idx] = e;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) mono[(size_t)(1 + k) * ldm + idx] = g[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mono[(size_t)(10 + k) * ldm + idx] = q[k];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) mono[(size_t)(13 + k) * ldm + idx] = gq[k];
+}
+
 // One configuration per wave, FP64; phases as k_ttm3f plus the gradient:
 // per-pair force slots summed per coordinate in a fixed partner order, the
 // dipole terms after the iteration, then the M-site distribution and the
-// charge-derivative terms per molecule.
+// charge-derivative terms per molecule.  The monomer terms come from
+// k_ttm3f_monomer_grad.
 __global__ void __launch_bounds__(64)
     k_ttm3f_grad(const double *__restrict__ crd, int64_t ldc, int nw, int64_t npts, double tol,
-                 SmearConst sc, double *__restrict__ E, double *__restrict__ G, int64_t ldg)
+                 SmearConst sc, const double *__restrict__ mono, int64_t ldm, double *__restrict__ E,
+                 double *__restrict__ G, int64_t ldg)
 {
     using namespace ttm;
     extern __shared__ double smem[];
@@ -845,13 +879,23 @@ __global__ void __launch_bounds__(64)
     for (int c = lane; c < 9 * nw; c += 64) x[c] = crd[(size_t)c * ldc + cfg];
     __syncthreads();
     double eint = 0.0;
+    const size_t m0 = (size_t)cfg * nw;
     if (lane < nw) {
         const int w = lane;
         const double *O = x + 9 * w, *H1 = O + 3, *H2 = O + 6;
 #pragma unroll
         for (int k = 0; k < 3; ++k) ms[3 * w + k] = gamma1 * O[k] + gamma2 * (H1[k] + H2[k]);
-        eint = ps_energy_grad(O, H1, H2, g + 9 * w);
-        dms_charges_grad(O, H1, H2, q + 3 * w, grdq + 27 * w);
+        eint = mono[m0 + w];
+    }
+    for (int e = lane; e < (kMonoFields - 1) * nw; e += 64) {
+        const int f = 1 + e / nw, w = e - (f - 1) * nw;
+        const double v = mono[(size_t)f * ldm + m0 + w];
+        if (f < 10)
+            g[9 * w + f - 1] = v;
+        else if (f < 13)
+            q[3 * w + f - 10] = v;
+        else
+            grdq[27 * w + f - 13] = v;
     }
     __syncthreads();
     double evdw = 0.0;
@@ -993,8 +1037,22 @@ hipError_t launch_ttm3f_grad(scp3g_ctx *ctx, int nw, int64_t npts, const double 
     if (nw < 1 || 3 * nw > 64) return hipErrorInvalidValue;
     const size_t smem = ttm3f_grad_smem_bytes(nw);
     if (smem > 64 * 1024) return hipErrorInvalidValue;
+    // monomer-term scratch, kept in the context and grown on demand
+    const int64_t ldm = npts * nw;
+    const size_t need = sizeof(double) * (size_t)kMonoFields * (size_t)ldm;
+    if (ctx->mono_bytes < need) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return e;
+        if (ctx->d_mono) hipFree(ctx->d_mono);
+        ctx->d_mono = nullptr;
+        ctx->mono_bytes = 0;
+        if ((e = hipMalloc(&ctx->d_mono, need)) != hipSuccess) return e;
+        ctx->mono_bytes = need;
+    }
+    hipLaunchKernelGGL(k_ttm3f_monomer_grad, dim3((unsigned)((ldm + 255) / 256)), dim3(256), 0, ctx->stream, crd,
+                       ldc, nw, npts, ctx->d_mono, ldm);
     hipLaunchKernelGGL(k_ttm3f_grad, dim3((unsigned)npts), dim3(64), smem, ctx->stream, crd, ldc, nw,
-                       npts, 1.0e-15, smear_const(), E, G, ldg);
+                       npts, 1.0e-15, smear_const(), (const double *)ctx->d_mono, ldm, E, G, ldg);
     return hipGetLastError();
 }
 

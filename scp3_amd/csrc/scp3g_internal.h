@@ -130,6 +130,9 @@ struct scp3g_ctx {
     bool timing = false;
     std::map<std::string, scp3g::KernelTimer> timers;
     std::vector<hipEvent_t> event_pool;
+    // TTM3-F gradient: per-(configuration, molecule) monomer terms
+    double *d_mono = nullptr;
+    size_t mono_bytes = 0;
 };
 
 // Sobol seeds skip..skip+npts-1 through the tile pipeline with weight mode
