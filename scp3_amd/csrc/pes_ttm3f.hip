@@ -848,14 +848,66 @@ __global__ void __launch_bounds__(256)
     for (int k = 0; k < 27; ++k) mono[(size_t)(13 + k) * ldm + idx] = gq[k];
 }
 
+// The charge and O-O pair terms of the gradient (ttm3f.cpp:403-560), one
+// lane per (configuration, molecule pair), for the same reason.
+// pairs[f * ldp + cfg * P + p]: f = 0 the pair's O-O energy, 1-12 its
+// potential/field slots, 13-17 its smeared dipole tensor, 18-35 its force
+// slots (pair_terms_grad's outputs, molecules i < j of pair p).
+constexpr int kPairFields = 36;
+
+__global__ void __launch_bounds__(256)
+    k_ttm3f_pair_grad(const double *__restrict__ crd, int64_t ldc, int nw, int64_t npts, SmearConst sc,
+                      const double *__restrict__ mono, int64_t ldm, double *__restrict__ pairs, int64_t ldp)
+{
+    using namespace ttm;
+    const int P = nw * (nw - 1) / 2;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= npts * P) return;
+    const int64_t cfg = idx / P;
+    const int p = (int)(idx - cfg * P);
+    int i = 0, rem = p;
+    while (rem >= nw - 1 - i) {
+        rem -= nw - 1 - i;
+        ++i;
+    }
+    const int j = i + 1 + rem;
+    // molecules i and j as molecules 0 and 1 of a two-molecule cluster
+    double x[18], ms[6], q[6], slot[12], tens[5], gs[18];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        x[k] = crd[(size_t)(9 * i + k) * ldc + cfg];
+        x[9 + k] = crd[(size_t)(9 * j + k) * ldc + cfg];
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            ms[3 * m + k] = gamma1 * x[9 * m + k] + gamma2 * (x[9 * m + 3 + k] + x[9 * m + 6 + k]);
+    const size_t m0 = (size_t)cfg * nw;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        q[k] = mono[(size_t)(10 + k) * ldm + m0 + i];
+        q[3 + k] = mono[(size_t)(10 + k) * ldm + m0 + j];
+    }
+    const double ev = pair_terms_grad(x, ms, q, 0, 1, sc, slot, tens, gs);
+    pairs[idx] = ev;
+#pragma unroll
+    for (int f = 0; f < 12; ++f) pairs[(size_t)(1 + f) * ldp + idx] = slot[f];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) pairs[(size_t)(13 + f) * ldp + idx] = tens[f];
+#pragma unroll
+    for (int f = 0; f < 18; ++f) pairs[(size_t)(18 + f) * ldp + idx] = gs[f];
+}
+
 // One configuration per wave, FP64; phases as k_ttm3f plus the gradient:
 // per-pair force slots summed per coordinate in a fixed partner order, the
 // dipole terms after the iteration, then the M-site distribution and the
 // charge-derivative terms per molecule.  The monomer terms come from
-// k_ttm3f_monomer_grad.
+// k_ttm3f_monomer_grad, the first pair terms from k_ttm3f_pair_grad.
 __global__ void __launch_bounds__(64)
     k_ttm3f_grad(const double *__restrict__ crd, int64_t ldc, int nw, int64_t npts, double tol,
-                 SmearConst sc, const double *__restrict__ mono, int64_t ldm, double *__restrict__ E,
+                 SmearConst sc, const double *__restrict__ mono, int64_t ldm,
+                 const double *__restrict__ pairs, int64_t ldp, double *__restrict__ E,
                  double *__restrict__ G, int64_t ldg)
 {
     using namespace ttm;
@@ -900,13 +952,14 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
     double evdw = 0.0;
     for (int p = lane; p < P; p += 64) {
-        int i = 0, rem = p;
-        while (rem >= nw - 1 - i) {
-            rem -= nw - 1 - i;
-            ++i;
-        }
-        const int j = i + 1 + rem;
-        evdw += pair_terms_grad(x, ms, q, i, j, sc, slots + 12 * p, tens + 5 * p, gsl + 18 * p);
+        const size_t pi = (size_t)cfg * P + p;
+        evdw += pairs[pi];
+#pragma unroll
+        for (int f = 0; f < 12; ++f) slots[12 * p + f] = pairs[(size_t)(1 + f) * ldp + pi];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) tens[5 * p + f] = pairs[(size_t)(13 + f) * ldp + pi];
+#pragma unroll
+        for (int f = 0; f < 18; ++f) gsl[18 * p + f] = pairs[(size_t)(18 + f) * ldp + pi];
     }
     __syncthreads();
     // forces of the charge-charge and O-O terms, per coordinate over partners
@@ -1037,9 +1090,12 @@ hipError_t launch_ttm3f_grad(scp3g_ctx *ctx, int nw, int64_t npts, const double 
     if (nw < 1 || 3 * nw > 64) return hipErrorInvalidValue;
     const size_t smem = ttm3f_grad_smem_bytes(nw);
     if (smem > 64 * 1024) return hipErrorInvalidValue;
-    // monomer-term scratch, kept in the context and grown on demand
-    const int64_t ldm = npts * nw;
-    const size_t need = sizeof(double) * (size_t)kMonoFields * (size_t)ldm;
+    // monomer- and pair-term scratch, kept in the context and grown on
+    // demand; configurations in chunks of at most ~1 GiB of scratch
+    const int P = nw * (nw - 1) / 2;
+    const size_t per_cfg = sizeof(double) * ((size_t)kMonoFields * nw + (size_t)kPairFields * P);
+    const int64_t chunk = std::max<int64_t>(256, std::min<int64_t>(npts, (int64_t)((1ull << 30) / per_cfg)));
+    const size_t need = per_cfg * (size_t)chunk;
     if (ctx->mono_bytes < need) {
         hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) return e;
@@ -1049,11 +1105,22 @@ hipError_t launch_ttm3f_grad(scp3g_ctx *ctx, int nw, int64_t npts, const double 
         if ((e = hipMalloc(&ctx->d_mono, need)) != hipSuccess) return e;
         ctx->mono_bytes = need;
     }
-    hipLaunchKernelGGL(k_ttm3f_monomer_grad, dim3((unsigned)((ldm + 255) / 256)), dim3(256), 0, ctx->stream, crd,
-                       ldc, nw, npts, ctx->d_mono, ldm);
-    hipLaunchKernelGGL(k_ttm3f_grad, dim3((unsigned)npts), dim3(64), smem, ctx->stream, crd, ldc, nw,
-                       npts, 1.0e-15, smear_const(), (const double *)ctx->d_mono, ldm, E, G, ldg);
-    return hipGetLastError();
+    const SmearConst sc = smear_const();
+    for (int64_t c0 = 0; c0 < npts; c0 += chunk) {
+        const int64_t nc = std::min<int64_t>(chunk, npts - c0);
+        const int64_t ldm = nc * nw, ldp = nc * P;
+        double *mono = ctx->d_mono, *pairs = ctx->d_mono + (size_t)kMonoFields * ldm;
+        hipLaunchKernelGGL(k_ttm3f_monomer_grad, dim3((unsigned)((ldm + 255) / 256)), dim3(256), 0, ctx->stream,
+                           crd + c0, ldc, nw, nc, mono, ldm);
+        if (P > 0)
+            hipLaunchKernelGGL(k_ttm3f_pair_grad, dim3((unsigned)((ldp + 255) / 256)), dim3(256), 0, ctx->stream,
+                               crd + c0, ldc, nw, nc, sc, (const double *)mono, ldm, pairs, ldp);
+        hipLaunchKernelGGL(k_ttm3f_grad, dim3((unsigned)nc), dim3(64), smem, ctx->stream, crd + c0, ldc, nw, nc,
+                           1.0e-15, sc, (const double *)mono, ldm, (const double *)pairs, ldp, E + c0, G + c0, ldg);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // Host FP64 evaluation of the same terms in the same order as the kernel
