@@ -89,13 +89,14 @@ def read_coord(path, n3):
     return x / S.BOHR, H
 
 
-def check_against_oracle(tmp_path, q0, mass, K0, pes, N, L, tol=1.0):
+def check_against_oracle(tmp_path, q0, mass, K0, pes, N, L, tol=1.0, cutoff=50.0):
     """tol scales every bound: the TTM3-F gradient kernel agrees with the
     oracle to ~1e-9 of the largest component (dipole iterations in another
     order), q-TIP4P/F to rounding."""
     npts = 1 << N
     skip = 1 << (N + 1)
-    out = S.scp1(q0, mass, K0, lambda q, d, iU: O.scp1_average(pes, q, d, iU, skip, npts), max_iterations=L)
+    out = S.scp1(q0, mass, K0, lambda q, d, iU: O.scp1_average(pes, q, d, iU, skip, npts), max_iterations=L,
+                 cutoff=cutoff / S.AUTOCM)
     F = np.loadtxt(tmp_path / "free_energy.dat", ndmin=1)
     Fo = np.array([h["F"] for h in out["history"]])
     np.testing.assert_allclose(F, Fo, rtol=1e-10 * tol)
@@ -173,3 +174,26 @@ def test_scp1_reproduces_reference_prism_free_energy(tmp_path):
     _, xs, _, _ = system.ohhohh(species, x, None)
     q, _ = read_coord(str(tmp_path / "coord.xyz"), xs.size)
     assert np.abs(q * S.BOHR - xs).max() < 1e-3
+
+
+def test_scp1_driver_cutoff_and_append(tmp_path):
+    """-c sets the frequency cutoff (SCP1_main.cpp:86-88); an input with a
+    Hessian opens omega_scp1.dat and free_energy.dat for appending
+    (SCP1.h:47,62-65), so a second run adds its iterations."""
+    print("scp1 driver: dimer qtip4pf -c 400 -N 10 -L 2, twice", flush=True)
+    path = system.input_path("dimer_qtip4pf")
+    r = run_scp1(["-c", "400", "-N", "10", "-L", "2", path], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    q0, mass, Hs = load("dimer_qtip4pf")
+    # modes below 400 cm^-1 leave the effective Hessian with zero eigenvalues
+    # beyond the six of translation and rotation; F = <V> + sum(omega[6:])/4
+    # (SCP1.cpp:165-173) then adds the square roots of their rounding noise,
+    # ~1e-10 Hartree that no two eigensolvers share
+    check_against_oracle(tmp_path, q0, mass, Hs, "qtip4pf", 10, 2, tol=300.0, cutoff=400.0)
+    r = run_scp1(["-c", "400", "-N", "10", "-L", "1", path], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    F = np.loadtxt(tmp_path / "free_energy.dat")
+    om = np.loadtxt(tmp_path / "omega_scp1.dat", ndmin=2)
+    assert F.size == 3 and om.shape[0] == 3
+    # the second run starts again from the input: its first iteration repeats
+    assert F[2] == F[0]
