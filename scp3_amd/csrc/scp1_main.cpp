@@ -1,7 +1,8 @@
 // SCP1: self-consistent phonons for a water cluster (SCP1_main.cpp,
 // SCP1.cpp), the Gaussian averages on the GPU through libscp3g.
 //
-//   SCP1 [-N log2(NSobol)] [-p PES] [-L max_iterations] [-c cutoff_cm-1] input
+//   SCP1 [-N log2(NSobol)] [-p PES] [-L max_iterations] [-c cutoff_cm-1]
+//        [-T temperature_K] input
 //
 // Same options and outputs as the reference: omega_scp1.dat (one line of
 // frequencies per iteration), free_energy.dat (F per iteration, Hartree),
@@ -9,6 +10,8 @@
 // format), "iter: n" per iteration and the final omega, <U>, <UX>, F and F0
 // on stdout.  -S and -r are accepted and, as in the reference (whose SCP1
 // object chooses its own skip and never reads the file), have no effect.
+// -T is an addition: the reference's main always passes kT = 0
+// (SCP1_main.cpp:132) although SCP1::operator() takes any temperature.
 // The averages replace SCP1::average (SCP1.cpp:33-86) with
 // scp3g_scp1_average: the whole Sobol range on one device instead of an MPI
 // split plus Allreduce.  The iteration (SCP1.cpp:89-190) stays on the host.
@@ -28,12 +31,14 @@
 namespace {
 
 constexpr double kAutokcalpmol = 627.509474277194;  // Constants.h:21
+constexpr double kAutoK = 315775.0428639057;         // Constants.cpp:14
 
 struct Options {
     int64_t nsobol = int64_t(1) << 20;  // SCP1_main.cpp:43
     std::string pes = "qtip4pf";
     int max_iterations = 200;
     double cutoff = -1.0;  // Hartree; <= 0 keeps 50 cm^-1 (SCP1.h:53)
+    double kT = 0.0;       // Hartree
     std::string input;
 };
 
@@ -52,10 +57,11 @@ Options parse(int argc, char **argv)
                                        {"PES", required_argument, nullptr, 'p'},
                                        {"max-itertions", required_argument, nullptr, 'L'},
                                        {"cutoff", required_argument, nullptr, 'c'},
+                                       {"temperature", required_argument, nullptr, 'T'},
                                        {nullptr, 0, nullptr, 0}};
     Options o;
     int ch;
-    while ((ch = getopt_long(argc, argv, "S:N:r:p:L:c:", longopts, nullptr)) != -1) {
+    while ((ch = getopt_long(argc, argv, "S:N:r:p:L:c:T:", longopts, nullptr)) != -1) {
         switch (ch) {
         case 'N': {
             const int p2 = std::atoi(optarg);
@@ -78,6 +84,12 @@ Options parse(int argc, char **argv)
         case 'c':
             o.cutoff = std::atof(optarg) / cli::autocm;
             break;
+        case 'T': {
+            const double T = std::atof(optarg);
+            if (!(T >= 0.0)) usage_exit("-T <K>: the temperature must be >= 0");
+            o.kT = T / kAutoK;
+            break;
+        }
         default:
             std::fprintf(stderr, "Unknown option: %d\n", ch);
             std::exit(EXIT_FAILURE);
@@ -285,7 +297,7 @@ int main(int argc, char **argv)
     // SCP1 constructor (SCP1.h:46-66)
     const bool append = !in.H.empty();
     const double cutoff = o.cutoff > 0 ? o.cutoff : 50.0 / cli::autocm;
-    const double qeps = 2.0, kT = 0.0;  // SCP1.h:55, SCP1_main.cpp:132
+    const double qeps = 2.0, kT = o.kT;  // SCP1.h:55; SCP1_main.cpp:132 passes 0
     const int64_t skip = int64_t(1) << ((int)std::floor(std::log2((double)o.nsobol)) + 1);
     FILE *omega_out = std::fopen("omega_scp1.dat", append ? "a" : "w");
     FILE *free_energy_out = std::fopen("free_energy.dat", append ? "a" : "w");
@@ -352,7 +364,10 @@ int main(int argc, char **argv)
             q[r] -= qeps * s;
         }
         F = Upot;  // :165-173
-        for (int i = 6; i < n; ++i) F += 0.25 * omega[i];
+        for (int i = 6; i < n; ++i)
+            F += kT > 0.0 ? kT * std::log(2.0 * std::sinh(0.5 / kT * omega[i])) -
+                                0.25 * omega[i] / std::tanh(0.5 / kT * omega[i])
+                          : 0.25 * omega[i];
         for (int i = 0; i < n; ++i) std::fprintf(omega_out, i ? " %.10e" : "%.10e", omega[i]);
         std::fprintf(omega_out, "\n");
         std::fprintf(free_energy_out, "%.17g\n", F);

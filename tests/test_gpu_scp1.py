@@ -89,14 +89,14 @@ def read_coord(path, n3):
     return x / S.BOHR, H
 
 
-def check_against_oracle(tmp_path, q0, mass, K0, pes, N, L, tol=1.0, cutoff=50.0):
+def check_against_oracle(tmp_path, q0, mass, K0, pes, N, L, tol=1.0, cutoff=50.0, kT=0.0):
     """tol scales every bound: the TTM3-F gradient kernel agrees with the
     oracle to ~1e-9 of the largest component (dipole iterations in another
     order), q-TIP4P/F to rounding."""
     npts = 1 << N
     skip = 1 << (N + 1)
     out = S.scp1(q0, mass, K0, lambda q, d, iU: O.scp1_average(pes, q, d, iU, skip, npts), max_iterations=L,
-                 cutoff=cutoff / S.AUTOCM)
+                 cutoff=cutoff / S.AUTOCM, kT=kT)
     F = np.loadtxt(tmp_path / "free_energy.dat", ndmin=1)
     Fo = np.array([h["F"] for h in out["history"]])
     np.testing.assert_allclose(F, Fo, rtol=1e-10 * tol)
@@ -197,3 +197,20 @@ def test_scp1_driver_cutoff_and_append(tmp_path):
     assert F.size == 3 and om.shape[0] == 3
     # the second run starts again from the input: its first iteration repeats
     assert F[2] == F[0]
+
+
+def test_scp1_driver_temperature(tmp_path):
+    """-T (an addition: the reference's main passes kT = 0) runs
+    SCP1::operator() at kT > 0: thermal widths and the harmonic free energy
+    (SCP1.cpp:18-31, 165-170).  One iteration: at 300 K the next one
+    drops soft modes below the cutoff, and kT log(2 sinh(omega / 2kT)) of
+    their zero frequencies is the logarithm of rounding noise, in the
+    reference as here."""
+    print("scp1 driver: dimer qtip4pf -T 300 -N 11 -L 1", flush=True)
+    path = system.input_path("dimer_qtip4pf")
+    r = run_scp1(["-T", "300", "-N", "11", "-L", "1", path], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    q0, mass, Hs = load("dimer_qtip4pf")
+    check_against_oracle(tmp_path, q0, mass, Hs, "qtip4pf", 11, 1, kT=300.0 / 315775.0428639057)
+    r = run_scp1(["-T", "-1", path], str(tmp_path))
+    assert r.returncode != 0 and "temperature" in r.stderr
