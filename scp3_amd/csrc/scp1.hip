@@ -275,7 +275,10 @@ int scp3g_scp1_average(scp3g_ctx *ctx, int pes, int nw, const double *q, const d
         rc = check_hip(ctx, hipGetLastError(), "scp1 fold");
     }
     timer_end(ctx, "scp1", ev);
-    if (rc) return rc;
+    if (rc) {
+        hipStreamSynchronize(ctx->stream);  // the host staging of the inputs is still read
+        return rc;
+    }
     std::vector<double> S(mm);
     if ((rc = check_hip(ctx, hipMemcpyAsync(S.data(), acc, mm * 8, hipMemcpyDeviceToHost, ctx->stream),
                         "download scp1 sums")) ||
