@@ -82,7 +82,8 @@ def parse():
     ap.add_argument("--cpu-chunk", type=int, default=8192, help="points per CPU baseline chunk")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cv-terms", type=int, default=1)
-    ap.add_argument("--scramble", type=int, default=0, help="Sobol digit scrambling seed (0 = reference sequence)")
+    ap.add_argument("--scramble", type=int, default=0, help="seed of the affine digit scramble of the Sobol points")
+    ap.add_argument("--no-scramble", action="store_true", help="the reference's unscrambled Sobol sequence")
     ap.add_argument("--mixed-kernel", type=int, default=None, help="SYRK kernel variant (scp3g_set_mixed_kernel)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
@@ -110,7 +111,7 @@ def setup(c, sysd, pr, mode, form=None):
     est = {"grad": scp3_amd.EST_GRADIENT, "stein": scp3_amd.EST_GRADIENT_STEIN}
     c.set_estimator(est.get((form or "").replace("_mixed", ""), scp3_amd.EST_ENERGY))
     c.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
-    c.set_scramble(pr.get("scramble", 0))
+    c.set_scramble(pr.get("scramble"))
     c.set_basis(sysd["modes"], sysd["nmodes2"], sysd["nmodes3"])
     c.set_pes("ttm3f" if "ttm3f" in pr["config"] else "qtip4pf")
     c.set_precision(mode)
@@ -325,7 +326,7 @@ def main():
     if a.tile is not None:
         pr["tile"] = a.tile
     pr["mixed_kernel"] = a.mixed_kernel
-    pr["scramble"] = a.scramble
+    pr["scramble"] = None if a.no_scramble else a.scramble
     sysd = system.prepare(system.input_path(pr["config"]), nmodes2=pr["n2"], nmodes3=pr["n3"])
     sysd["name"] = pr["config"]
     P = 1 << pr["points"]

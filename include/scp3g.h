@@ -113,15 +113,17 @@ int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast);
  */
 int scp3g_set_estimator(scp3g_ctx *ctx, int estimator);
 /*
- * Scrambled Sobol points: seed != 0 applies a random affine matrix scramble
- * of the binary digits to every dimension (Matousek 1998: a lower-triangular
- * binary matrix with unit diagonal and a digital shift, drawn from seed; the
- * class of scramble(p, 'MatousekAffineOwen') behind the reference's -r
- * files, SCP3.cpp:188-194); 0 (default) restores the reference's sequence
- * (sobol.cpp:14198).  Needs scp3g_set_system first; applies to the context's
- * accumulate calls, not to scp3g_sobol_normals.
+ * Scrambled Sobol points: scrambled != 0 applies a random affine matrix
+ * scramble of the binary digits to every dimension (Matousek 1998: a
+ * lower-triangular binary matrix with unit diagonal and a digital shift,
+ * drawn from `seed`, any value including 0; the class of
+ * scramble(p, 'MatousekAffineOwen') behind the reference's -r files,
+ * SCP3.cpp:188-194); scrambled == 0 (the default) is the reference's
+ * unscrambled sequence (sobol.cpp:14198), seed ignored.  Needs
+ * scp3g_set_system first (which keeps the setting); applies to the
+ * context's accumulate calls, not to scp3g_sobol_normals.
  */
-int scp3g_set_scramble(scp3g_ctx *ctx, uint64_t seed);
+int scp3g_set_scramble(scp3g_ctx *ctx, int scrambled, uint64_t seed);
 /* points per device tile (multiple of 256), or 0 (default): the largest power
  * of two <= 524288 whose FP64 basis table fits in 8 GiB, at least 4096 */
 int scp3g_set_tile(scp3g_ctx *ctx, int tile);

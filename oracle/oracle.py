@@ -41,6 +41,8 @@ def lib():
         L = ctypes.CDLL(LIB)
         L.orc_sobol_ints.argtypes = [I, I64, I64, P]
         L.orc_sobol_normals.argtypes = [I, I64, I64, I, P]
+        L.orc_sobol_normals_scr.argtypes = [I, I64, I64, I, I, ctypes.c_uint64, P]
+        L.orc_sobol_scramble.argtypes = [I, ctypes.c_uint64, I64, P]
         L.orc_sobol_table.argtypes = [I, P]
         L.orc_bsm.argtypes = [I64, P, P]
         L.orc_ho.argtypes = [I, D]
@@ -114,11 +116,23 @@ def sobol_ints(dims, seed0, npts):
     return out
 
 
-def sobol_normals(dims, seed0, npts, halve=True):
+def sobol_normals(dims, seed0, npts, halve=True, scramble=None):
+    """Normals of seeds seed0.. (points x dims); scramble = a seed for the
+    affine digit scramble (None: the reference's sequence)."""
     y = np.zeros((npts, dims))
-    rc = lib().orc_sobol_normals(dims, seed0, npts, 1 if halve else 0, _p(y))
+    if scramble is None:
+        rc = lib().orc_sobol_normals(dims, seed0, npts, 1 if halve else 0, _p(y))
+    else:
+        rc = lib().orc_sobol_normals_scr(dims, seed0, npts, 1 if halve else 0, 1, int(scramble), _p(y))
     assert rc == 0, rc
     return y
+
+
+def sobol_scramble(ints, seed):
+    """Affine digit scramble of Sobol integers (points x dims), a copy."""
+    x = np.ascontiguousarray(ints, dtype=np.uint64).copy()
+    lib().orc_sobol_scramble(x.shape[1], int(seed), x.shape[0], _p(x))
+    return x
 
 
 def bsm(u):

@@ -125,6 +125,53 @@ int orc_sobol_ints(int dims, int64_t seed0, int64_t npts, uint64_t *out)
     return 0;
 }
 
+/*
+ * Affine digit scrambling (no reference counterpart: the reference reads
+ * scrambled normals from a -r file made by MATLAB, SCP3.cpp:188-194).  Per
+ * dimension d a lower-triangular binary matrix L with unit diagonal acts
+ * on the 62 digits (digit 0 = bit 61, the most significant) and a digital
+ * shift e is added: x -> L x + e over GF(2).  Row i of L is drawn from
+ * splitmix64 (state seed ^ (0x9e3779b97f4a7c15 (d + 1))): bit 61-i set,
+ * the bits above it random, the bits below it zero; then one draw for e.
+ * Restated on the integers of a point (the device applies L to the
+ * direction integers instead, which is the same map because L is linear).
+ */
+static uint64_t orc_splitmix64(uint64_t *st)
+{
+    uint64_t z = (*st += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+static void orc_scramble_rows(int d, uint64_t seed, uint64_t *rows, uint64_t *shift)
+{
+    const uint64_t all = ((uint64_t)1 << ORC_MAXCOL) - 1;
+    uint64_t st = seed ^ (0x9e3779b97f4a7c15ull * (uint64_t)(d + 1));
+    for (int i = 0; i < ORC_MAXCOL; i++) {
+        const int b = ORC_MAXCOL - 1 - i;
+        const uint64_t above = all & ~((((uint64_t)2) << b) - 1);
+        rows[i] = ((uint64_t)1 << b) | (orc_splitmix64(&st) & above);
+    }
+    *shift = orc_splitmix64(&st) & all;
+}
+
+/* out[i*dims + d] = L_d x[i*dims + d] + e_d in place */
+void orc_sobol_scramble(int dims, uint64_t seed, int64_t npts, uint64_t *x)
+{
+    uint64_t rows[ORC_MAXCOL], e;
+    for (int d = 0; d < dims; d++) {
+        orc_scramble_rows(d, seed, rows, &e);
+        for (int64_t p = 0; p < npts; p++) {
+            const uint64_t v = x[(size_t)p * dims + d];
+            uint64_t o = 0;
+            for (int i = 0; i < ORC_MAXCOL; i++)
+                o |= (uint64_t)(__builtin_popcountll(v & rows[i]) & 1) << (ORC_MAXCOL - 1 - i);
+            x[(size_t)p * dims + d] = o ^ e;
+        }
+    }
+}
+
 /* Beasley-Springer-Moro inverse normal, beasley_springer_moro.cpp:14-46 */
 void orc_bsm(int64_t n, const double *u, double *x)
 {
@@ -156,12 +203,14 @@ void orc_bsm(int64_t n, const double *u, double *x)
  * divides by sqrt(2).  Output point-major y[i*dims + d] (the column of
  * `by` for point i), already divided by sqrt(2) when halve != 0.
  */
-int orc_sobol_normals(int dims, int64_t seed0, int64_t npts, int halve, double *y)
+int orc_sobol_normals_scr(int dims, int64_t seed0, int64_t npts, int halve, int scrambled,
+                          uint64_t sseed, double *y)
 {
     uint64_t *q = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)dims * (size_t)npts);
     if (!q) return -2;
     int rc = orc_sobol_ints(dims, seed0, npts, q);
     if (rc) { free(q); return rc; }
+    if (scrambled) orc_sobol_scramble(dims, sseed, npts, q);
     const double recipd = 1.0 / (double)((uint64_t)1 << 62);     /* :27780 */
     size_t tot = (size_t)dims * (size_t)npts;
     for (size_t k = 0; k < tot; k++) y[k] = (double)q[k] * recipd;
@@ -172,6 +221,11 @@ int orc_sobol_normals(int dims, int64_t seed0, int64_t npts, int halve, double *
     }
     free(q);
     return 0;
+}
+
+int orc_sobol_normals(int dims, int64_t seed0, int64_t npts, int halve, double *y)
+{
+    return orc_sobol_normals_scr(dims, seed0, npts, halve, 0, 0, y);
 }
 
 /* ------------------------------------------------------------------ */

@@ -48,7 +48,7 @@ _SIGS = {
     "scp3g_set_mixed_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_cv_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "scp3g_set_estimator": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
-    "scp3g_set_scramble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    "scp3g_set_scramble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64]),
     "scp3g_hessian": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                      ctypes.c_double, ctypes.c_void_p]),
     "scp3g_scp1_average": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
@@ -210,8 +210,12 @@ class Context:
         self.check(self.L.scp3g_set_estimator(self._h, int(est)))
 
     def set_scramble(self, seed):
-        """Affine digit scrambling of the Sobol points (seed != 0), 0 = reference sequence."""
-        self.check(self.L.scp3g_set_scramble(self._h, int(seed)))
+        """Affine digit scrambling of the Sobol points drawn from `seed` (any
+        integer, 0 included); None restores the reference's sequence."""
+        if seed is None:
+            self.check(self.L.scp3g_set_scramble(self._h, 0, 0))
+        else:
+            self.check(self.L.scp3g_set_scramble(self._h, 1, int(seed)))
 
     def set_mixed_kernel(self, kernel):
         """0: 64 x 64 direct-load SYRKs; 1, 2: LDS-tiled SYRKs (see scp3g.h)"""

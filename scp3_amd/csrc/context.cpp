@@ -183,6 +183,8 @@ static void free_sums(scp3g_ctx *ctx)
     dfree(ctx->d_F);
     dfree(ctx->d_Heff);
     ctx->h_n = 0;
+    ctx->points_done = 0;
+    ctx->norm_sum = 0.0;
 }
 
 // Points per tile when the caller sets none: per-point kernels need >= 64k
@@ -365,7 +367,8 @@ int scp3g_set_system(scp3g_ctx *ctx, int nw, const double *x0, int nmodes0,
     ctx->MUa.assign(MUa, MUa + (size_t)n3 * nmodes0);
     std::vector<uint64_t> dirs((size_t)nmodes0 * kSobolCols + nmodes0, 0);
     sobol_directions(nmodes0, dirs.data());
-    if (ctx->scramble) sobol_scramble(nmodes0, ctx->scramble, dirs.data(), dirs.data() + (size_t)nmodes0 * kSobolCols);
+    if (ctx->scramble_on)
+        sobol_scramble(nmodes0, ctx->scramble, dirs.data(), dirs.data() + (size_t)nmodes0 * kSobolCols);
     dfree(ctx->d_sobol);
     dfree(ctx->d_x0);
     dfree(ctx->d_omega);
@@ -381,9 +384,15 @@ int scp3g_set_system(scp3g_ctx *ctx, int nw, const double *x0, int nmodes0,
     rc = check_hip(ctx, hipMemcpy(ctx->d_MUa, MUa, ctx->MUa.size() * 8, hipMemcpyHostToDevice),
                    "upload system");
     if (rc) return rc;
-    // the work buffers depend on nw / nmodes0
+    // the work buffers depend on nw / nmodes0, the running sums on the
+    // system; the basis stays only if every selected mode still exists
+    // (a mode list keeps the caller's order, so check each entry)
     free_work(ctx);
-    if (ctx->modes.empty() || (int)ctx->modes.back() >= nmodes0) {
+    free_sums(ctx);
+    grad_release(ctx);
+    bool keep = !ctx->modes.empty();
+    for (int m : ctx->modes) keep = keep && m >= 0 && m < nmodes0;
+    if (!keep) {
         ctx->modes.clear();
         ctx->nbasis = 0;
     }
@@ -500,7 +509,7 @@ int scp3g_set_estimator(scp3g_ctx *ctx, int estimator)
     return 0;
 }
 
-int scp3g_set_scramble(scp3g_ctx *ctx, uint64_t seed)
+int scp3g_set_scramble(scp3g_ctx *ctx, int scrambled, uint64_t seed)
 {
     if (!ctx) return -1;
     if (ctx->nmodes0 <= 0) return fail(ctx, -2, "system not set (scp3g_set_system)");
@@ -508,12 +517,15 @@ int scp3g_set_scramble(scp3g_ctx *ctx, uint64_t seed)
     const int M = ctx->nmodes0;
     std::vector<uint64_t> dirs((size_t)M * kSobolCols + M, 0);
     sobol_directions(M, dirs.data());
-    if (seed) sobol_scramble(M, seed, dirs.data(), dirs.data() + (size_t)M * kSobolCols);
+    if (scrambled) sobol_scramble(M, seed, dirs.data(), dirs.data() + (size_t)M * kSobolCols);
     int rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (!rc)
         rc = check_hip(ctx, hipMemcpy(ctx->d_sobol, dirs.data(), dirs.size() * 8, hipMemcpyHostToDevice),
                        "upload scrambled directions");
-    if (!rc) ctx->scramble = seed;
+    if (!rc) {
+        ctx->scramble_on = scrambled != 0;
+        ctx->scramble = seed;
+    }
     return rc;
 }
 
@@ -819,6 +831,7 @@ int scp3g_accumulate_energies(scp3g_ctx *ctx, int64_t skip, int64_t npts, const 
 {
     int rc = ready(ctx);
     if (rc) return rc;
+    SyncOnError guard(ctx, rc);
     if (npts < 0 || skip < 0 || (!E_host && npts > 0)) return fail(ctx, -4, "bad arguments");
     if (skip + npts + 64 * 16 >= ((int64_t)1 << 60)) return fail(ctx, -4, "SEED too large");
     hipSetDevice(ctx->device);
@@ -846,6 +859,7 @@ int scp3g_accumulate_gradients(scp3g_ctx *ctx, int64_t skip, int64_t npts, const
 {
     int rc = ready(ctx);
     if (rc) return rc;
+    SyncOnError guard(ctx, rc);
     if (npts < 0 || skip < 0 || ((!E_host || !G_host) && npts > 0)) return fail(ctx, -4, "bad arguments");
     if (ctx->estimator == SCP3G_EST_ENERGY)
         return fail(ctx, -4, "scp3g_accumulate_gradients needs a gradient estimator (scp3g_set_estimator)");

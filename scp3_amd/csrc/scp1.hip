@@ -226,6 +226,8 @@ int scp3g_scp1_average(scp3g_ctx *ctx, int pes, int nw, const double *q, const d
     double *acc = part + (size_t)partial_chunks(w.T) * mm;
 
     std::vector<double> aux(2 * (size_t)n3 + (size_t)n3 * n3);
+    // declared after aux: an error return waits for its upload before aux goes
+    SyncOnError sync_guard(ctx, rc);
     for (int k = 0; k < n3; ++k) {
         aux[k] = std::sqrt(d[k]);
         aux[n3 + k] = q[k];
@@ -275,10 +277,7 @@ int scp3g_scp1_average(scp3g_ctx *ctx, int pes, int nw, const double *q, const d
         rc = check_hip(ctx, hipGetLastError(), "scp1 fold");
     }
     timer_end(ctx, "scp1", ev);
-    if (rc) {
-        hipStreamSynchronize(ctx->stream);  // the host staging of the inputs is still read
-        return rc;
-    }
+    if (rc) return rc;
     std::vector<double> S(mm);
     if ((rc = check_hip(ctx, hipMemcpyAsync(S.data(), acc, mm * 8, hipMemcpyDeviceToHost, ctx->stream),
                         "download scp1 sums")) ||

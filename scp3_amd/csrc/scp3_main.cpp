@@ -45,6 +45,7 @@ struct Opts {
     std::string rng_file, pes = "qtip4pf", input, modes_desc;
     int n2 = 0, n3 = 0, bw = 1024, levels = 10, device = 0;
     bool gradient = true, have_modes = false, cpu_pes = false, stein = false;
+    bool scrambled = false;
     unsigned long long scramble = 0;
     int precision = SCP3G_FP64;
 };
@@ -127,7 +128,10 @@ Opts parse(int argc, char **argv)
         case 1002: o.device = atoi(optarg); break;
         case 1003: o.cpu_pes = true; break;
         case 1004: o.stein = true; break;
-        case 1005: o.scramble = strtoull(optarg, nullptr, 0); break;
+        case 1005:
+            o.scrambled = true;
+            o.scramble = strtoull(optarg, nullptr, 0);
+            break;
         default: die("Unknown option: " + std::to_string(ch));
         }
     }
@@ -184,7 +188,7 @@ int main(int argc, char **argv)
     if (scp3g_create(&ctx, o.device)) die("no usable GPU");
     check(ctx, scp3g_set_system(ctx, N / 3, in.x0.data(), nm0, omega.data(), MUa.data()), "set_system");
     check(ctx, scp3g_set_basis(ctx, (int)modes.size(), modes.data(), o.n2, o.n3), "set_basis");
-    if (o.scramble) check(ctx, scp3g_set_scramble(ctx, o.scramble), "set_scramble");
+    if (o.scrambled) check(ctx, scp3g_set_scramble(ctx, 1, o.scramble), "set_scramble");
     if (!o.cpu_pes) check(ctx, scp3g_set_pes(ctx, o.pes.c_str()), "set_pes");
     check(ctx, scp3g_set_precision(ctx, o.precision), "set_precision");
     if (o.gradient)

@@ -70,7 +70,9 @@ struct scp3g_ctx {
     bool cv_one_term = false;
     // SCP3G_EST_ENERGY (the -G estimator) or one of the gradient forms
     int estimator = SCP3G_EST_ENERGY;
-    // Sobol digit scrambling seed, 0 = the reference's sequence
+    // Sobol digit scrambling: on/off (off = the reference's sequence) and
+    // the seed of the random matrices and shifts (0 is a seed like any other)
+    bool scramble_on = false;
     uint64_t scramble = 0;
 
     // device copies of the set-up
@@ -144,6 +146,19 @@ namespace scp3g {
 
 int fail(scp3g_ctx *ctx, int code, const std::string &msg);
 int check_hip(scp3g_ctx *ctx, hipError_t e, const char *what);
+
+// Waits for the context's stream when a call returns an error: host staging
+// buffers handed to asynchronous uploads (pageable vectors, the caller's
+// arrays) must not be released while a copy from them may still be queued.
+struct SyncOnError {
+    scp3g_ctx *ctx;
+    const int &rc;
+    SyncOnError(scp3g_ctx *c, const int &r) : ctx(c), rc(r) {}
+    ~SyncOnError()
+    {
+        if (rc) hipStreamSynchronize(ctx->stream);
+    }
+};
 
 // timing helpers: bracket a launch on ctx->stream
 void timer_begin(scp3g_ctx *ctx, const char *name, hipEvent_t *ev0);

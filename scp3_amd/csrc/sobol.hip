@@ -165,7 +165,7 @@ hipError_t launch_sobol(scp3g_ctx *ctx, const uint64_t *dirs, int dims, int64_t 
     dim3 grid((unsigned)((npts + per_block - 1) / per_block), (unsigned)dims);
     // the context's own table carries the digital shifts of a scrambled
     // sequence behind its direction integers (scp3g_set_scramble)
-    const uint64_t *shift = (dirs == ctx->d_sobol && ctx->scramble) ? dirs + (size_t)dims * kSobolCols : nullptr;
+    const uint64_t *shift = (dirs == ctx->d_sobol && ctx->scramble_on) ? dirs + (size_t)dims * kSobolCols : nullptr;
     hipLaunchKernelGGL(k_sobol, grid, dim3(64 * waves), 0, ctx->stream, dirs, shift, skip, npts,
                        halve, y, ldy, ints);
     return hipGetLastError();

@@ -162,6 +162,38 @@ def test_ttm3f_gradient_matches_oracle(ctx, name):
         np.testing.assert_allclose(Gg[i], g, rtol=0, atol=1e-9 * np.max(np.abs(g)))
 
 
+def test_ttm3f_gradient_across_scratch_chunks(ctx):
+    """scp3g_pes_gradient runs TTM3-F configurations in chunks sized by its
+    ~1 GiB monomer and pair scratch (17567 configurations for 20 waters:
+    8 x (40 nw + 36 nw(nw-1)/2) bytes each).  35634 configurations cross two
+    chunk boundaries with a ragged end; configurations on both sides of each
+    boundary and spread over the range match the oracle."""
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path("w20_ttm3f"), nmodes2=0, nmodes3=0)
+    n3 = sysd["x0"].size
+    nw = n3 // 9
+    per_cfg = 8 * (40 * nw + 36 * nw * (nw - 1) // 2)
+    chunk = (1 << 30) // per_cfg
+    P = 2 * chunk + 500
+    print("ttm3f gradient chunks: nw %d, %d configurations, chunk %d" % (nw, P, chunk), flush=True)
+    rng = np.random.default_rng(7)
+    X = sysd["x0"][None, :] * scp3_amd.BOHR + rng.normal(size=(P, n3)) * 0.03
+    crd = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    E = torch.zeros(P, dtype=torch.float64, device="cuda")
+    G = torch.zeros(n3 * P, dtype=torch.float64, device="cuda")
+    ctx.check(ctx.L.scp3g_pes_gradient(ctx.handle, scp3_amd.PES_TTM3F, nw, P, vp(crd), P, vp(E), vp(G), P))
+    ctx.sync()
+    Eg = E.cpu().numpy()
+    Gg = G.cpu().numpy().reshape(n3, P)
+    check = set(range(0, P, 997)) | {P - 1}
+    for b in (chunk, 2 * chunk):
+        check |= set(range(b - 3, b + 3))
+    for i in sorted(check):
+        e, g = oracle.ttm3f_grad(X[i])
+        assert Eg[i] == pytest.approx(e, rel=1e-12, abs=1e-10), i
+        np.testing.assert_allclose(Gg[:, i], g, rtol=0, atol=1e-9 * np.max(np.abs(g)), err_msg=str(i))
+
+
 def test_grad_sums_hexamer_ttm3f(ctx):
     """The gradient estimator with TTM3-F (the oracle's TTM3-F gradient)."""
     from scp3_amd import system

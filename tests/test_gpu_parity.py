@@ -242,6 +242,39 @@ def test_accumulate_normals_matches_sobol_path(ctx, dimer):
     assert mx < 1e-9
 
 
+@pytest.mark.parametrize("mode", [scp3_amd.FP64, scp3_amd.MIXED])
+def test_harmonic_potential_identity(ctx, hexamer, mode):
+    """Harmonic-PES identity (SCP3.cpp:219-223,237-239,269-271): energies
+    E = autokcalpmol * 1/2 y.(omega o y) make every weight bV zero, so Mout
+    is the HO diagonal plus the unselected modes' zero-point energy and the
+    levels are E0 + sum_k n_k omega_k.  The diagonal is built here from the
+    state list, not from the oracle.  A -m subset of 12 of the 48 modes with
+    doubles and triples (147 states) covers every diagonal family."""
+    from test_oracle import state_list
+    print("harmonic identity precision", mode, flush=True)
+    modes = np.array(list(range(10)) + [20, 30], dtype=np.int32)
+    ctx.set_system(hexamer["x0"], hexamer["omega"], hexamer["MUa"])
+    ctx.set_basis(modes, 12, 6)
+    ctx.set_pes("qtip4pf")
+    ctx.set_precision(mode)
+    ctx.set_tile(2048)
+    skip, P = 1 << 30, 5000
+    y = oracle.sobol_normals(48, skip, P)
+    E = scp3_amd.AUTOKCALPMOL * 0.5 * (y * hexamer["omega"] * y).sum(1)
+    ctx.reset()
+    ctx.accumulate_energies(skip, E)
+    om = hexamer["omega"]
+    om_sel = om[modes]
+    sl = state_list(modes.size, 12, 6)
+    ho = np.array([0.5 * om_sel.sum() + sum(om_sel[k] for k in t) for t in sl])
+    ho += 0.5 * (om.sum() - om_sel.sum())
+    Mout = ctx.h_download()
+    assert Mout.shape == (len(sl), len(sl)) == (147, 147)
+    assert np.abs(Mout - np.diag(ho)).max() < 1e-14 * ho.max()
+    ev = ctx.lowest_levels(10, 1e-11)
+    np.testing.assert_allclose(ev, np.sort(ho)[:10], rtol=0, atol=1e-12)
+
+
 def test_mixed_matches_fp64_within_budget(ctx, hexamer):
     P = 1 << 15
     out = {}
@@ -422,60 +455,44 @@ def test_two_contexts_in_two_threads(dimer, hexamer):
         np.testing.assert_array_equal(out[tag], out[tag + "_alone"])
 
 
-M64 = (1 << 64) - 1
-
-
-def _splitmix64(st):
-    st = (st + 0x9e3779b97f4a7c15) & M64
-    z = st
-    z = ((z ^ (z >> 30)) * 0xbf58476d1ce4e5b9) & M64
-    z = ((z ^ (z >> 27)) * 0x94d049bb133111eb) & M64
-    return st, z ^ (z >> 31)
-
-
-def _scramble_ints(x, seed):
-    """Affine digit scramble L x + e per dimension (the library's documented
-    generator), restated on the oracle's Sobol integers (points x dims)."""
-    out = np.zeros_like(x)
-    allb = (1 << 62) - 1
-    for d in range(x.shape[1]):
-        st = (seed ^ ((0x9e3779b97f4a7c15 * (d + 1)) & M64)) & M64
-        masks = []
-        for i in range(62):
-            b = 61 - i
-            st, r = _splitmix64(st)
-            masks.append((1 << b) | (r & allb & ~((2 << b) - 1)))
-        st, e = _splitmix64(st)
-        e &= allb
-        col = []
-        for v in x[:, d].tolist():
-            o = 0
-            for i, m in enumerate(masks):
-                o |= (bin(v & m).count("1") & 1) << (61 - i)
-            col.append(o ^ e)
-        out[:, d] = np.array(col, dtype=np.uint64)
-    return out
+def test_set_system_revalidates_selected_modes(ctx, dimer):
+    """A mode list keeps the caller's order (SCP3 -m "7,0-3"): a new system
+    keeps the basis only while every selected mode exists, and a system
+    change clears the running sum."""
+    setup_system(ctx, dimer, tile=1024)
+    ctx.set_basis(np.array([7, 0, 1, 2, 3], dtype=np.int32), 5, 0)
+    n = ctx.basis_size()
+    ctx.reset()
+    ctx.accumulate(1 << 30, 512)
+    assert ctx.points_done() == 512
+    ctx.set_system(dimer["x0"], dimer["omega"], dimer["MUa"])
+    assert ctx.basis_size() == n and ctx.points_done() == 0
+    ctx.set_system(dimer["x0"], dimer["omega"][:6], dimer["MUa"][:, :6])
+    assert ctx.basis_size() == 0
+    with pytest.raises(scp3_amd.Scp3Error):
+        ctx.accumulate(1 << 30, 64)
 
 
 def test_scrambled_points(ctx, dimer):
     """scp3g_set_scramble: the points are L x + e of the reference's Sobol
-    integers (checked through the mapped Cartesians), differ between seeds,
-    and seed 0 restores the reference sequence."""
+    integers (the oracle's restatement, checked through the mapped
+    Cartesians) for seed 0 and another seed, the two seeds differ, and
+    switching scrambling off restores the reference sequence."""
     setup_system(ctx, dimer, tile=1024)
-    P, skip, seed = 700, 1 << 30, 0x5eed
+    P, skip = 700, 1 << 30
     M0 = dimer["omega"].size
-    ctx.set_scramble(seed)
-    X = ctx.points_coords(skip, P)
-    ints = _scramble_ints(oracle.sobol_ints(M0, skip, P), seed)
-    u = ints.astype(np.float64) * 2.0 ** -62
-    y = oracle.bsm(u.ravel()).reshape(u.shape) / np.sqrt(2.0)
-    ref = scp3_amd.BOHR * (dimer["x0"][None, :] + y @ dimer["MUa"].T)
-    assert np.abs(X - ref).max() < 1e-11 * np.abs(ref).max()
-    ctx.set_scramble(seed + 1)
-    X2 = ctx.points_coords(skip, P)
-    assert np.abs(X2 - X).max() > 1e-3
-    ctx.set_scramble(0)
+    X = {}
+    for seed in (0, 0x5eed):
+        print("scrambled points seed", seed, flush=True)
+        ctx.set_scramble(seed)
+        X[seed] = ctx.points_coords(skip, P)
+        y = oracle.sobol_normals(M0, skip, P, scramble=seed)
+        ref = scp3_amd.BOHR * (dimer["x0"][None, :] + y @ dimer["MUa"].T)
+        assert np.abs(X[seed] - ref).max() < 1e-11 * np.abs(ref).max()
+    assert np.abs(X[0x5eed] - X[0]).max() > 1e-3
+    ctx.set_scramble(None)
     X0 = ctx.points_coords(skip, P)
     y0 = oracle.sobol_normals(M0, skip, P)
     ref0 = scp3_amd.BOHR * (dimer["x0"][None, :] + y0 @ dimer["MUa"].T)
     assert np.abs(X0 - ref0).max() < 1e-11 * np.abs(ref0).max()
+    assert np.abs(X0 - X[0]).max() > 1e-3

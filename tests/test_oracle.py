@@ -83,6 +83,70 @@ def test_sobol_is_stratified(dims):
             assert np.array_equal(np.sort(top[:, d]), np.arange(4096))
 
 
+# ---- affine digit scrambling -------------------------------------------------
+
+M64 = (1 << 64) - 1
+
+
+def _splitmix64(st):
+    st = (st + 0x9e3779b97f4a7c15) & M64
+    z = st
+    z = ((z ^ (z >> 30)) * 0xbf58476d1ce4e5b9) & M64
+    z = ((z ^ (z >> 27)) * 0x94d049bb133111eb) & M64
+    return st, z ^ (z >> 31)
+
+
+def scramble_ints_py(x, seed):
+    """Affine digit scramble L x + e per dimension, bit by bit in Python:
+    row i of L (output digit i, digit 0 = bit 61) has bit 61-i set and
+    random bits above it from splitmix64 seeded seed ^ (0x9e37... (d+1));
+    one more draw gives e."""
+    out = np.zeros_like(x)
+    allb = (1 << 62) - 1
+    for d in range(x.shape[1]):
+        st = (seed ^ ((0x9e3779b97f4a7c15 * (d + 1)) & M64)) & M64
+        masks = []
+        for i in range(62):
+            b = 61 - i
+            st, r = _splitmix64(st)
+            masks.append((1 << b) | (r & allb & ~((2 << b) - 1)))
+        st, e = _splitmix64(st)
+        e &= allb
+        col = []
+        for v in x[:, d].tolist():
+            o = 0
+            for i, m in enumerate(masks):
+                o |= (bin(v & m).count("1") & 1) << (61 - i)
+            col.append(o ^ e)
+        out[:, d] = np.array(col, dtype=np.uint64)
+    return out
+
+
+@pytest.mark.parametrize("seed", [0, 1, 0x5eed])
+def test_scramble_restatement_and_net_property(seed):
+    """The oracle's affine digit scramble equals the bitwise restatement, is
+    affine (s(x) ^ s(y) ^ s(0) = s(x ^ y)), keeps each dimension's first 2^m
+    points a permutation of the 2^m dyadic intervals (the unit lower
+    triangle maps digit prefixes one to one), and seed 0 is a real scramble."""
+    x = oracle.sobol_ints(5, 0, 256)
+    sx = oracle.sobol_scramble(x, seed)
+    np.testing.assert_array_equal(sx, scramble_ints_py(x, seed))
+    rng = np.random.default_rng(seed + 1)
+    a = rng.integers(0, 1 << 62, size=(40, 5), dtype=np.uint64)
+    b = rng.integers(0, 1 << 62, size=(40, 5), dtype=np.uint64)
+    z = oracle.sobol_scramble(np.zeros((1, 5), dtype=np.uint64), seed)
+    np.testing.assert_array_equal(oracle.sobol_scramble(a, seed) ^ oracle.sobol_scramble(b, seed) ^ z,
+                                  oracle.sobol_scramble(a ^ b, seed))
+    for m in (4, 8):
+        top = sx[: 1 << m] >> np.uint64(62 - m)
+        for d in range(5):
+            assert sorted(top[:, d].tolist()) == list(range(1 << m))
+    assert np.mean(sx == x) < 0.05
+    y = oracle.sobol_normals(5, 0, 256, scramble=seed)
+    u = sx.astype(np.float64) * 2.0 ** -62
+    np.testing.assert_array_equal(y, oracle.bsm(u.ravel()).reshape(u.shape) / np.sqrt(2.0))
+
+
 # ---- Beasley-Springer-Moro -------------------------------------------------
 
 def test_bsm_matches_reference_bitwise():
@@ -158,6 +222,37 @@ def test_get_bra_order_and_values(M, M2, M3):
     assert L.orc_sub_basis_size(M, M2, M3, 1) == 1 + M
     assert L.orc_sub_basis_size(M, M2, M3, 2) == 1 + M + M2 * (M2 + 1) // 2
     assert L.orc_sub_basis_size(M, M2, M3, 3) == n
+
+
+@pytest.mark.parametrize("M,M2,M3,sel", [(12, 12, 0, None), (7, 5, 4, None), (9, 9, 9, None),
+                                         (12, 4, 2, [0, 1, 2, 3, 7])])
+def test_ho_diagonal_brute_force(M, M2, M3, sel):
+    """addHODiagonal (ScaledMatrixElements.cpp:735-784) and the zero-point
+    energy of unselected modes (SCP3.cpp:238,270), built independently from
+    the state list of python/base_frequencies.py:42-60: state t holds
+    quanta n_k = t.count(k) and has E0 + sum_k n_k omega_sel[k] with
+    E0 = 1/2 sum omega_sel; finalize adds 1/2 (sum omega - sum omega_sel).
+    A swapped (k,k,l)/(k,l,l) pair, E0 over all modes or a lost unselected
+    ZPE fails one of the cases."""
+    rng = np.random.default_rng(M + 10 * M2 + 100 * M3)
+    omega = np.sort(rng.uniform(1e-4, 2e-2, size=M))
+    modes = np.arange(M, dtype=np.int32) if sel is None else np.array(sel, dtype=np.int32)
+    om_sel = omega[modes]
+    sl = state_list(modes.size, M2, M3)
+    n = len(sl)
+    ho = np.array([0.5 * om_sel.sum() + sum(om_sel[k] for k in t) for t in sl])
+    D = np.zeros((n, n), order="F")
+    oracle.add_ho_diagonal(modes.size, M2, M3, om_sel, D)
+    assert not np.any(D - np.diag(np.diag(D)))       # the diagonal only
+    np.testing.assert_allclose(np.diag(D), ho, rtol=2e-15, atol=0)
+    zpe_rest = 0.5 * (omega.sum() - om_sel.sum())
+    if sel is not None:
+        assert zpe_rest > 0
+    sysd = dict(modes=modes, omega=omega, nmodes2=M2, nmodes3=M3)
+    P = 37
+    raw = np.asfortranarray(np.triu(rng.normal(size=(n, n))))
+    out = oracle.finalize(sysd, raw, P)
+    np.testing.assert_allclose(out, raw / P + np.diag(ho + zpe_rest), rtol=1e-14, atol=1e-16)
 
 
 def test_basis_orthonormal_under_gauss_hermite():
