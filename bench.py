@@ -55,7 +55,12 @@ PEAK_BF16_TFLOPS = 2500.0
 PRESETS = {
     "c1": dict(config="dimer_qtip4pf", n2=12, n3=0, points=16, tile=0, levels=5, forms="fp64,mixed"),
     "c2": dict(config="hexamer_qtip4pf", n2=48, n3=0, points=20, tile=0, levels=10, forms="fp64,mixed"),
-    "c3": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=0, levels=10, forms="fp64,mixed"),
+    # the default: the plain FP64 and mixed forms over 2^21 points per step
+    # (FP64 capped at 2 timed steps, 15 s each), and the control variate as
+    # c4 runs it (2^24 surrogate + 2^21 true points, one timed step)
+    "c3": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=0, levels=10, forms="mixed,fp64,cv",
+               cv_nn=24, cv_t=21, mlp_hidden=256, mlp_steps=20000, mlp_batch=4096, mlp_pairs=18,
+               caps={"fp64": (2, 1), "cv": (1, 1)}),
     "c4": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=0, levels=10, forms="fp64,cv",
                cv_nn=24, cv_t=19, mlp_hidden=256, mlp_steps=20000, mlp_batch=4096, mlp_pairs=18),
     "c5": dict(config="w20_ttm3f", n2=174, n3=0, points=20, tile=0, levels=20, forms="fp64,cv",
@@ -74,17 +79,26 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--preset", default="c2", choices=sorted(PRESETS))
+    ap.add_argument("--preset", default="c3", choices=sorted(PRESETS))
     ap.add_argument("--points", type=int, default=None, help="log2 of Sobol points per step")
     ap.add_argument("--forms", default=None)
     ap.add_argument("--tile", type=int, default=None, help="points per tile (0 = automatic)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline time budget")
-    ap.add_argument("--cpu-chunk", type=int, default=8192, help="points per CPU baseline chunk")
+    ap.add_argument("--cpu-chunk", type=int, default=1024,
+                    help="points per CPU baseline block (the reference's -B, SCP3.cpp:58)")
+    ap.add_argument("--converge-summary", nargs="+", default=None,
+                    help="JSON-line outputs of --converge runs: write the convergence record of the preset")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cv-terms", type=int, default=1)
     ap.add_argument("--scramble", type=int, default=0, help="seed of the affine digit scramble of the Sobol points")
     ap.add_argument("--no-scramble", action="store_true", help="the reference's unscrambled Sobol sequence")
     ap.add_argument("--mixed-kernel", type=int, default=None, help="SYRK kernel variant (scp3g_set_mixed_kernel)")
+    ap.add_argument("--converge", default=None, help="convergence study of one form (fp64, mixed, stein_mixed, cv)")
+    ap.add_argument("--min-log2", type=int, default=21)
+    ap.add_argument("--max-log2", type=int, default=24)
+    ap.add_argument("--seeds", default="0", help="scramble seeds of the convergence study")
+    ap.add_argument("--level-tol", type=float, default=1e-7, help="Davidson residual tolerance (hartree)")
+    ap.add_argument("--cv-t", type=int, default=None, help="log2 of true-potential points of the control variate")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     return ap.parse_args()
@@ -118,6 +132,79 @@ def setup(c, sysd, pr, mode, form=None):
     c.set_tile(pr["tile"])
     if pr.get("mixed_kernel") is not None:
         c.set_mixed_kernel(pr["mixed_kernel"])
+
+
+def converge(a, pr, sysd):
+    """Convergence study of one form: levels after a growing number of Sobol
+    points (one running sum, levels read at each checkpoint; cv: the
+    correction term over 2^cv_t true-potential points once, the surrogate
+    term grown over the checkpoints), one JSON line per checkpoint with the
+    accumulate wall time up to it."""
+    import scp3_amd
+    form = a.converge
+    mode = scp3_amd.FP64 if form in ("fp64", "grad", "stein") else scp3_amd.MIXED
+    k = pr["levels"]
+    skip = 1 << 30
+    cps = []
+    for m in range(a.min_log2, a.max_log2 + 1):
+        cps.append(1 << m)
+        if m < a.max_log2:
+            cps.append(3 << (m - 1))
+    with scp3_amd.Context(0) as c:
+        c.L.scp3g_set_cv_terms(c.handle, a.cv_terms)
+        for seed in [int(s) for s in a.seeds.split(",")]:
+            pr["scramble"] = seed
+            setup(c, sysd, pr, mode, form)
+            train_s = 0.0
+            if form == "cv":
+                c.sync()
+                t0 = time.perf_counter()
+                c.mlp_init(pr["mlp_hidden"], seed=0)
+                c.mlp_sample_data(1 << 40, 1 << pr["mlp_pairs"])
+                loss = c.mlp_train(pr["mlp_steps"], pr["mlp_batch"], 1e-3)
+                c.sync()
+                train_s = time.perf_counter() - t0
+                print(json.dumps({"form": form, "seed": seed, "train_s": train_s,
+                                  "final_loss": float(loss[-100:].mean())}), flush=True)
+            c.reset()
+            done, t_acc, Ct = 0, 0.0, None
+            if form == "cv":
+                # the correction term once over 2^cv_t true-potential points
+                # (the first of the surrogate's seeds), then the surrogate
+                # term grown checkpoint by checkpoint in its own sum:
+                # H = S / P_nn + C / P_t
+                Pt = 1 << pr["cv_t"]
+                c.sync()
+                t0 = time.perf_counter()
+                c.accumulate_correction(skip, Pt)
+                c.sync()
+                t_acc = time.perf_counter() - t0
+                Ct = c.h_download(raw=True) / Pt
+                c.reset()
+            for P in cps:
+                c.sync()
+                t0 = time.perf_counter()
+                if form == "cv":
+                    c.accumulate_surrogate(skip + done, P - done)
+                else:
+                    c.accumulate(skip + done, P - done)
+                c.sync()
+                t_acc += time.perf_counter() - t0
+                done = P
+                t1 = time.perf_counter()
+                if form == "cv":
+                    M = c.h_download()
+                    M += Ct
+                    ev = c.matrix_lowest_levels(M, k, a.level_tol)
+                    del M
+                else:
+                    ev = c.lowest_levels(k, a.level_tol)
+                t_lev = time.perf_counter() - t1
+                print(json.dumps({"form": form, "seed": seed, "points": P,
+                                  "points_t": (1 << pr["cv_t"]) if form == "cv" else None,
+                                  "accumulate_s": round(t_acc, 4), "train_s": round(train_s, 4),
+                                  "eigensolve_s": round(t_lev, 4),
+                                  "levels_cm1": [round(float(v) * scp3_amd.AUTOCM, 4) for v in ev]}), flush=True)
 
 
 def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
@@ -190,46 +277,49 @@ def pmc_traffic(kernel, config, n, points_per_launch):
     return None
 
 
-def cpu_baseline(sysd, skip, chunk, target_s, max_points):
-    """FP64 oracle path on the host over a bounded prefix of the same points:
-    chunks of `chunk` points until `target_s` seconds or `max_points`."""
+def cpu_baseline(sysd, pes, skip, scramble, bw, target_s, P_total):
+    """The -G path as SCP3 runs it on the host, on the same scrambled points:
+    per block of bw points (the reference's -B, SCP3.cpp:58) the sampler
+    (oracle C Sobol + BSM), the serial potential loop of SCP3.cpp:205-225
+    with the reference's own potential (oracle/_ref, else the oracle's
+    restatement), get_bra under OpenMP over the points
+    (ScaledMatrixElements.cpp:956-961) and dsyr2k('U', 'N', 0.5, bra,
+    bra diag(bV), 1, M) in OpenBLAS (scipy's).  Blocks run until target_s;
+    the rate is extrapolated linearly to P_total points."""
     import scipy.linalg.blas as blas
 
     from oracle import oracle
-    from scp3_amd import AUTOKCALPMOL, BOHR
-    M0 = sysd["omega"].size
     modes = sysd["modes"]
+    M0 = sysd["omega"].size
     n = oracle.basis_size(modes.size, sysd["nmodes2"], sysd["nmodes3"])
-    cores = len(os.sched_getaffinity(0))
-    pes = oracle.ttm3f if "ttm3f" in sysd["name"] else oracle.qtip4pf
-    t = dict(sample=0.0, map=0.0, pes=0.0, basis=0.0, syrk=0.0)
-    Mm = np.zeros((n, n), order="F")
+    fnptr, kind = oracle.pes_pointer(pes)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    t = dict(sample=0.0, pes=0.0, basis=0.0, syr2k=0.0)
+    M = np.zeros((n, n), order="F")
+    bra = np.zeros((bw, n))
     done = 0
-    while done < max_points and sum(t.values()) < target_s:
-        S = chunk
+    while done < P_total and sum(t.values()) < target_s:
         t0 = time.perf_counter()
-        y = oracle.sobol_normals(M0, skip + done, S)
+        y = oracle.sobol_normals(M0, skip + done, bw, scramble=scramble)
         t1 = time.perf_counter()
-        r = BOHR * (sysd["x0"][None, :] + y @ sysd["MUa"].T)
+        bV = oracle.block_weights(sysd, fnptr, y)
         t2 = time.perf_counter()
-        V = np.array([pes(r[i]) for i in range(S)]) / AUTOKCALPMOL
-        bV = V - 0.5 * (y * sysd["omega"] * y).sum(1)
+        oracle.block_bra(sysd, y, out=bra)
         t3 = time.perf_counter()
-        bra = oracle.get_bra(y[:, modes], modes.size, sysd["nmodes2"], sysd["nmodes3"])
+        ket = bra * bV[:, None]
+        M = blas.dsyr2k(0.5, bra.T, ket.T, beta=1.0, c=M, trans=0, lower=0, overwrite_c=1)
         t4 = time.perf_counter()
-        A = np.asfortranarray(bra.T)
-        Bk = np.asfortranarray(bra.T * bV[None, :])
-        Mm = blas.dsyr2k(0.5, A, Bk, beta=1.0, c=Mm, trans=0, lower=0, overwrite_c=1)
-        t5 = time.perf_counter()
-        for k, dt in zip(t, (t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4)):
-            t[k] += dt
-        done += S
+        for key, dt in zip(t, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
+            t[key] += dt
+        done += bw
     total = sum(t.values())
-    return dict(value=done / total, unit="points/s", cores=cores, kind="port",
-                sample="%d Sobol points (skip 2^30, %d-point chunks, %.1f s) of the same workload; "
-                       "phases %s s; FP64 oracle path: C Sobol/BSM, numpy map, serial potential loop, "
-                       "get_bra, OpenBLAS dsyr2k (%d threads)"
-                       % (done, chunk, total, {k: round(v, 3) for k, v in t.items()}, cores))
+    rate = done / total
+    return dict(value=rate, unit="points/s", cores=threads, kind="port",
+                sample="%d scrambled Sobol points (seed %s, skip 2^30) of the same workload in %d-point blocks "
+                       "(%.1f s): sampler %.2f s, serial %s potential loop (%s build) %.2f s, get_bra %.2f s, "
+                       "OpenBLAS dsyr2k %.2f s; %d threads for get_bra and dsyr2k; extrapolated linearly "
+                       "to %d points: %.0f s" % (done, scramble, bw, total, t["sample"], pes, kind, t["pes"],
+                                                 t["basis"], t["syr2k"], threads, P_total, P_total / rate))
 
 
 def scp1_modes(path):
@@ -303,6 +393,87 @@ def bench_scp1(a, pr):
     print(json.dumps(out))
 
 
+CONV_DIR = os.path.join(ROOT, "scp3_amd", "data")
+
+
+def convergence_path(pr, n):
+    return os.path.join(CONV_DIR, "convergence_%s_%dstates.json" % (pr["config"], n))
+
+
+def write_convergence(pr, files, n):
+    """Convergence record of a workload from --converge outputs: the
+    reference levels (mean over scramble seeds of the last checkpoint of the
+    stein_mixed runs, with the standard error of that mean) and, for each
+    -G form, every checkpoint's largest level deviation from them and the
+    first checkpoint from which all levels stay within 0.5 cm^-1."""
+    rows = []
+    for fn in files:
+        rows += [json.loads(l) for l in open(fn) if l.strip().startswith("{")]
+    rows = [r for r in rows if "points" in r]
+    st = [r for r in rows if r["form"] == "stein_mixed"]
+    pmax = max(r["points"] for r in st)
+    L = np.array([r["levels_cm1"] for r in st if r["points"] == pmax])
+    ref = L.mean(0)
+    sem = L.std(0, ddof=1) / np.sqrt(L.shape[0]) if L.shape[0] > 1 else np.zeros_like(ref)
+    rec = {"workload": "%s, %d states, scrambled Sobol points (skip 2^30)" % (pr["config"], n),
+           "reference": {"estimator": "gradient (Stein), mixed form", "points": pmax,
+                         "seeds": sorted(r["seed"] for r in st if r["points"] == pmax),
+                         "levels_cm1": [round(float(v), 4) for v in ref],
+                         "sem_cm1": [round(float(v), 4) for v in sem]},
+           "forms": {}}
+    for form in sorted({r["form"] for r in rows} - {"stein_mixed"}):
+        cps = sorted((r for r in rows if r["form"] == form), key=lambda r: r["points"])
+        table = []
+        for r in cps:
+            dev = float(np.abs(np.array(r["levels_cm1"]) - ref).max())
+            table.append({"points": r["points"], "points_t": r.get("points_t"), "seed": r["seed"],
+                          "accumulate_s": r["accumulate_s"], "train_s": r.get("train_s", 0.0),
+                          "max_dev_cm1": round(dev, 3)})
+        req = None
+        for q in range(len(table)):
+            if all(t["max_dev_cm1"] <= 0.5 for t in table[q:]):
+                req = table[q]["points"]
+                break
+        rec["forms"][form] = {"checkpoints": table, "points_required": req}
+    path = convergence_path(pr, n)
+    with open(path, "w") as f:
+        json.dump(rec, f, indent=1)
+    print("wrote", path, file=sys.stderr)
+
+
+def time_to_half_wavenumber(pr, res, n):
+    """Time to the lowest levels within 0.5 cm^-1 of the converged levels:
+    the points each form needs (committed convergence record) at this run's
+    measured rates.  The FP64 form is the mixed form's estimator on the same
+    points (levels equal to 0.011 cm^-1), so it needs the mixed form's
+    points.  None when no record exists for the workload."""
+    path = convergence_path(pr, n)
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path))
+    out = {"record": os.path.relpath(path, ROOT), "reference_points": rec["reference"]["points"],
+           "reference_sem_cm1_max": max(rec["reference"]["sem_cm1"])}
+    plain = rec["forms"].get("mixed", {}).get("points_required")
+    for f in ("fp64", "mixed"):
+        if f in res and plain:
+            rr = res[f]
+            rate = rr["points"] / rr["t_acc"]
+            out[f] = {"points": plain, "points_per_s": rate, "s": plain / rate + rr["t_lev"]}
+    cvr = rec["forms"].get("cv", {}).get("points_required")
+    if "cv" in res and cvr and "mixed" in res:
+        rr = res["cv"]
+        Pt = 1 << pr["cv_t"]
+        t_t = Pt / (res["mixed"]["points"] / res["mixed"]["t_acc"])
+        rate_nn = (1 << pr["cv_nn"]) / max(rr["t_acc"] - t_t, 1e-9)
+        out["cv"] = {"points": cvr, "points_t": Pt, "surrogate_points_per_s": rate_nn,
+                     "train_s": rr["train_s"], "s": rr["train_s"] + cvr / rate_nn + t_t + rr["t_lev"]}
+    if "fp64" in out and "cv" in out:
+        out["speedup_cv_vs_fp64"] = out["fp64"]["s"] / out["cv"]["s"]
+    if "fp64" in out and "mixed" in out:
+        out["speedup_mixed_vs_fp64"] = out["fp64"]["s"] / out["mixed"]["s"]
+    return out
+
+
 def main():
     a = parse()
     if a.copy_bytes:
@@ -329,19 +500,35 @@ def main():
     pr["scramble"] = None if a.no_scramble else a.scramble
     sysd = system.prepare(system.input_path(pr["config"]), nmodes2=pr["n2"], nmodes3=pr["n3"])
     sysd["name"] = pr["config"]
+    if a.cv_t is not None:
+        pr["cv_t"] = a.cv_t
+    if a.converge_summary:
+        from oracle import oracle
+        n = oracle.basis_size(sysd["modes"].size, sysd["nmodes2"], sysd["nmodes3"])
+        write_convergence(pr, a.converge_summary, n)
+        return
+    if a.converge:
+        converge(a, pr, sysd)
+        return
     P = 1 << pr["points"]
     skip = 1 << 30
     forms = pr["forms"].split(",")
-    res, extra = {}, {}
+    caps = pr.get("caps", {})
+    res, extra, nsteps = {}, {}, {}
     with scp3_amd.Context(0) as c:
         c.L.scp3g_set_cv_terms(c.handle, a.cv_terms)
         for f in forms:
-            print("bench: preset %s form %s" % (a.preset, f), file=sys.stderr, flush=True)
-            res[f] = run_form(c, sysd, pr, f, P, skip, a.steps, a.warmup, pr["levels"], extra)
+            st, wu = a.steps, a.warmup
+            if f in caps and f != forms[0]:
+                st, wu = min(st, caps[f][0]), min(wu, caps[f][1])
+            nsteps[f] = (st, wu)
+            print("bench: preset %s form %s (%d steps, %d warm-up)" % (a.preset, f, st, wu),
+                  file=sys.stderr, flush=True)
+            res[f] = run_form(c, sysd, pr, f, P, skip, st, wu, pr["levels"], extra)
             print("bench: form %s accumulate %.4f s, levels %.4f s" % (f, res[f]["t_acc"], res[f]["t_lev"]),
                   file=sys.stderr, flush=True)
     n = res[forms[0]]["n"]
-    main_form = "mixed" if "mixed" in res else ("cv" if "cv" in res else forms[-1])
+    main_form = forms[0]
     r = res[main_form]
     # dominant kernel: the SYRK's MFMA kernel (split-K partial fold timed
     # apart); algorithmic flops of a symmetric rank-K update: n(n+1) per point
@@ -349,11 +536,12 @@ def main():
     avg_s = ms / 1e3 / max(cnt, 1)
     # points_done counts one step; the gradient forms run two SYRKs per tile
     # (FP64) or one over twice the points (mixed)
-    pts_per_launch = r["points"] * a.steps / max(cnt, 1) * (2 if main_form in GRAD_FORMS else 1)
+    pts_per_launch = r["points"] * nsteps[main_form][0] / max(cnt, 1) * (2 if main_form in GRAD_FORMS else 1)
     flops = float(n) * (n + 1) * pts_per_launch
     fp64_main = main_form in ("fp64", "grad", "stein")
     peak = PEAK_FP64_TFLOPS if fp64_main else PEAK_BF16_TFLOPS
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
+    scr = "scrambled (seed %d)" % pr["scramble"] if pr["scramble"] is not None else "unscrambled"
     out = {
         "metric": "Sobol pts/s into H + time to lowest-10 levels at 0.5 cm^-1, GPU forms vs plain",
         "value": r["points"] / r["t_acc"],
@@ -366,44 +554,48 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if fp64_main else "f64 H; bf16x3 MFMA SYRK, f32 potential/basis",
-        "data": "synthetic: seed-built relaxed cluster (scp3_amd/data), Sobol points",
-        "config": {"workload": "%s: %s, %d states, 2^%d Sobol points, %d-point tiles, lowest %d levels"
+        "data": "synthetic: seed-built relaxed cluster (scp3_amd/data), %s Sobol points" % scr,
+        "config": {"workload": "%s: %s, %d states, 2^%d Sobol points per step, %d-point tiles, lowest %d levels"
                                % (a.preset, pr["config"], n, pr["points"], round(pts_per_launch), pr["levels"]),
                    "basis_states": n, "points": P, "form": main_form},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
                      "traffic": pmc_traffic("k_syrk_x3<3>", pr["config"], n, pts_per_launch)
                      if main_form == "mixed" else None,
-                     "kernel": "%s (%s form)" % ("k_syrk_f64_lds" if fp64_main else "k_syrk_x3", main_form),
+                     "kernel": "%s (%s form)" % ("k_syrk_f64_lds" if fp64_main else "k_syrk_x3<3>", main_form),
                      "avg_launch_ms": avg_s * 1e3,
                      "flops_per_launch": flops},
         "forms": {},
     }
     for f, rr in res.items():
         out["forms"][f] = {
+            "steps": nsteps[f][0],
+            "warmup": nsteps[f][1],
             "points": rr["points"],
             "points_per_s": rr["points"] / rr["t_acc"],
             "time_to_levels_s": rr["t_acc"] + rr["t_lev"] + rr["train_s"],
             "accumulate_s": rr["t_acc"],
             "eigensolve_s": rr["t_lev"],
             "levels_cm1": [round(float(v) * scp3_amd.AUTOCM, 4) for v in rr["levels"]],
-            "kernel_ms_per_step": {kk: round(v[0] / a.steps, 4) for kk, v in rr["kernels"].items() if v[1]},
+            "kernel_ms_per_step": {kk: round(v[0] / nsteps[f][0], 4) for kk, v in rr["kernels"].items() if v[1]},
         }
         if f == "cv":
             out["forms"][f].update(extra)
+            out["forms"][f]["points_surrogate"] = 1 << pr["cv_nn"]
+            out["forms"][f]["points_t"] = 1 << pr["cv_t"]
     base = main_form.replace("_mixed", "")
     if main_form.endswith("_mixed") and base in res:
         d = np.abs(res[main_form]["levels"] - res[base]["levels"]).max() * scp3_amd.AUTOCM
         out["max_level_diff_vs_%s_cm1" % base] = float(d)
-        out["speedup_time_to_levels_vs_%s" % base] = (out["forms"][base]["time_to_levels_s"]
-                                                      / out["forms"][main_form]["time_to_levels_s"])
     if "fp64" in res and main_form != "fp64":
         d = np.abs(res[main_form]["levels"] - res["fp64"]["levels"]).max() * scp3_amd.AUTOCM
         out["max_level_diff_vs_fp64_cm1"] = float(d)
-        t64 = out["forms"]["fp64"]["time_to_levels_s"]
-        out["speedup_time_to_levels_vs_fp64"] = t64 / out["forms"][main_form]["time_to_levels_s"]
+    tt = time_to_half_wavenumber(pr, res, n)
+    if tt:
+        out["time_to_0p5_cm1"] = tt
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sysd, skip, a.cpu_chunk, a.cpu_seconds, P)
+        out["cpu_baseline"] = cpu_baseline(sysd, "ttm3f" if "ttm3f" in pr["config"] else "qtip4pf", skip,
+                                           pr["scramble"], a.cpu_chunk, a.cpu_seconds, P)
     print(json.dumps(out))
 
 

@@ -247,6 +247,12 @@ int scp3g_set_cv_terms(scp3g_ctx *ctx, int terms);
  * skip..skip+npts-1, normaliser += npts (QMC with V_nn in place of V, the
  * first term of scp3g_accumulate_cv); no reference counterpart */
 int scp3g_accumulate_surrogate(scp3g_ctx *ctx, int64_t skip, int64_t npts);
+/* the correction term alone: H += sum B B^T (bV - w_nn) over Sobol seeds
+ * skip..skip+npts-1 with the true potential, normaliser += npts (QMC of the
+ * residual V - V_nn, the second term of scp3g_accumulate_cv); with
+ * scp3g_accumulate_surrogate in another context it lets a caller grow the
+ * two terms separately; no reference counterpart */
+int scp3g_accumulate_correction(scp3g_ctx *ctx, int64_t skip, int64_t npts);
 
 /* ---- per-stage entry points (device pointers; used by parity tests) ---- */
 

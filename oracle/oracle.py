@@ -67,6 +67,8 @@ def lib():
         L.orc_ttm3f_grad.restype = D
         L.orc_scp3_accumulate_grad.argtypes = [I, I, P, P, P, I, P, I, I, I, I, I64, I64, P, P, P, P]
         L.orc_scp1_average.argtypes = [I, I, P, P, P, I64, I64, P, P, P]
+        L.orc_block_weights.argtypes = [I, I, P, P, P, P, I64, P, P]
+        L.orc_block_bra.argtypes = [I, P, I, I, I, I64, P, P]
         if hasattr(L, "orc_ttm3f"):
             try:
                 L.orc_ttm3f.argtypes = [I64, P]
@@ -296,3 +298,40 @@ def finalize(sysd, Mraw, npts):
     add_ho_diagonal(modes.size, sysd["nmodes2"], sysd["nmodes3"], om_sel, M)
     M[np.diag_indices_from(M)] += 0.5 * (sysd["omega"].sum() - om_sel.sum())
     return M
+
+
+def pes_pointer(name, reference=True):
+    """Address of a C energy function with the PES.h signature (nw, crd) ->
+    kcal/mol: the reference's own build (oracle/_ref) when present and
+    `reference`, else this oracle's restatement.  Returns (address, kind)."""
+    R = ref() if reference else None
+    if R is not None:
+        return ctypes.cast(getattr(R, "ref_" + name), ctypes.c_void_p).value, "reference"
+    return ctypes.cast(getattr(lib(), "orc_" + name), ctypes.c_void_p).value, "port"
+
+
+def block_weights(sysd, fnptr, y):
+    """bV of a block of halved normals y (points x nmodes0), the serial
+    potential loop of SCP3.cpp:205-225 through the C function fnptr."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    bV = np.zeros(y.shape[0])
+    x0 = np.ascontiguousarray(sysd["x0"], dtype=np.float64)
+    om = np.ascontiguousarray(sysd["omega"], dtype=np.float64)
+    MU = np.asfortranarray(sysd["MUa"], dtype=np.float64)
+    rc = lib().orc_block_weights(sysd["nw"], om.size, _p(x0), _p(om), _p(MU), ctypes.c_void_p(fnptr),
+                                 y.shape[0], _p(y), _p(bV))
+    assert rc == 0, rc
+    return bV
+
+
+def block_bra(sysd, y, out=None):
+    """Basis rows of a block (points x n), get_bra under OpenMP."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    modes = np.ascontiguousarray(sysd["modes"], dtype=np.int32)
+    n = basis_size(modes.size, sysd["nmodes2"], sysd["nmodes3"])
+    if out is None:
+        out = np.zeros((y.shape[0], n))
+    rc = lib().orc_block_bra(modes.size, _p(modes), y.shape[1], sysd["nmodes2"], sysd["nmodes3"],
+                             y.shape[0], _p(y), _p(out))
+    assert rc == 0, rc
+    return out

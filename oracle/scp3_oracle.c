@@ -506,3 +506,65 @@ int orc_scp3_accumulate(int nw, int nmodes0, const double *x0, const double *ome
     free(bV);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* CPU baseline pieces (timed by bench.py's cpu_baseline leg, never by   */
+/* the product): SCP3_a's block loop split where the reference splits   */
+/* its parallelism                                                       */
+/* ------------------------------------------------------------------ */
+
+typedef double (*orc_pes_ext)(int64_t nw, const double *crd);
+
+/*
+ * The serial point loop of SCP3.cpp:205-225 for one block: r = bohr (x0 +
+ * MUa y) (:208), V = pot(nw, r) / autokcalpmol (:219, any potential with
+ * the PES.h energy signature, e.g. the reference's own), bV = V - 1/2
+ * y.(omega o y) (:222-223).  y is point-major (npts x nmodes0), halved.
+ */
+int orc_block_weights(int nw, int nmodes0, const double *x0, const double *omega, const double *MUa,
+                      orc_pes_ext fn, int64_t npts, const double *y, double *bV)
+{
+    const int n3 = 9 * nw;
+    double *r = (double *)malloc(sizeof(double) * n3);
+    if (!r || !fn) { free(r); return -4; }
+    for (int64_t j = 0; j < npts; j++) {
+        const double *yj = y + (size_t)j * nmodes0;
+        for (int c = 0; c < n3; c++) {
+            double s = 0.0;
+            for (int k = 0; k < nmodes0; k++) s += MUa[(size_t)k * n3 + c] * yj[k];
+            r[c] = ORC_BOHR * (x0[c] + s);
+        }
+        const double V = fn(nw, r) / ORC_AUTOKCALPMOL;
+        double h = 0.0;
+        for (int k = 0; k < nmodes0; k++) h += yj[k] * (omega[k] * yj[k]);
+        bV[j] = V - 0.5 * h;
+    }
+    free(r);
+    return 0;
+}
+
+/*
+ * get_bra over a block of points with OpenMP over the points, as addEpot
+ * does (ScaledMatrixElements.cpp:956-961): bra + j n receives the row of
+ * point j (the column of the reference's n x bw `bra`).
+ */
+int orc_block_bra(int nsel, const int *modes, int nmodes0, int nmodes2, int nmodes3, int64_t npts,
+                  const double *y, double *bra)
+{
+    const int64_t n = orc_basis_size(nsel, nmodes2, nmodes3);
+    int bad = 0;
+#pragma omp parallel
+    {
+        double *q = (double *)malloc(sizeof(double) * (nsel > 0 ? nsel : 1));
+        if (!q) bad = 1;
+#pragma omp for schedule(static)
+        for (int64_t j = 0; j < npts; j++) {
+            if (!q) continue;
+            const double *yj = y + (size_t)j * nmodes0;
+            for (int m = 0; m < nsel; m++) q[m] = yj[modes[m]];
+            orc_get_bra(nsel, nmodes2, nmodes3, q, bra + (size_t)j * n);
+        }
+        free(q);
+    }
+    return bad ? -2 : 0;
+}

@@ -58,6 +58,7 @@ _SIGS = {
                                           ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_int64]),
     "scp3g_accumulate_surrogate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
+    "scp3g_accumulate_correction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "scp3g_points_coords": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
     "scp3g_accumulate_energies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
     "scp3g_accumulate_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
@@ -303,6 +304,14 @@ class Context:
         loss = np.zeros(1, dtype=np.float32)
         self.check(self.L.scp3g_mlp_grad(self._h, batch, _ptr(g), _ptr(loss)))
         return g, float(loss[0])
+
+    def accumulate_surrogate(self, skip, npts):
+        """surrogate term alone: sum B B^T w_nn over Sobol seeds skip.., normaliser += npts"""
+        self.check(self.L.scp3g_accumulate_surrogate(self._h, int(skip), int(npts)))
+
+    def accumulate_correction(self, skip, npts):
+        """correction term alone: sum B B^T (bV - w_nn), normaliser += npts"""
+        self.check(self.L.scp3g_accumulate_correction(self._h, int(skip), int(npts)))
 
     def accumulate_cv(self, skip_nn, P_nn, skip_t, P_t):
         self.check(self.L.scp3g_accumulate_cv(self._h, int(skip_nn), int(P_nn), int(skip_t), int(P_t)))

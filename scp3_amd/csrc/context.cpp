@@ -182,6 +182,8 @@ static void free_sums(scp3g_ctx *ctx)
     dfree(ctx->d_Hc);
     dfree(ctx->d_F);
     dfree(ctx->d_Heff);
+    dfree(ctx->d_mout);
+    dfree(ctx->d_diag);
     ctx->h_n = 0;
     ctx->points_done = 0;
     ctx->norm_sum = 0.0;
@@ -559,8 +561,9 @@ int scp3g_reset(scp3g_ctx *ctx)
 // one tile: y already in ctx->d_y (nmodes0 x tile, halved), np points.
 // wmode 0: plain weights bV (SCP3.cpp:219-223); 1: surrogate weights w_nn;
 // 2: control-variate correction bV - w_nn; 3: plain weights from energies
-// the caller put in ctx->d_E.  Modes 1 and 2 multiply the weights by
-// `scale`; the normaliser grows by np for modes 0 and 3, np*scale for 1.
+// the caller put in ctx->d_E; 5: the correction bV - w_nn as a sum of its
+// own.  Modes 1, 2 and 5 multiply the weights by `scale`; the normaliser
+// grows by np for modes 0, 3 and 5, np*scale for 1.
 }  // extern "C"
 
 // The gradient estimator's tile (SCP3.cpp:205-231 with use_gradient):
@@ -695,7 +698,7 @@ int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
     timer_end(ctx, "pes", ev);
     if (rc) return rc;
 weights:
-    if (wmode == 2) {
+    if (wmode == 2 || wmode == 5) {
         timer_begin(ctx, "mlp", &ev);
         rc = check_hip(ctx, launch_mlp_weights(ctx, ctx->d_y, T, np, ctx->d_E, 1, scale, ctx->d_w), "mlp");
         timer_end(ctx, "mlp", ev);
@@ -736,7 +739,7 @@ basis:
         if (rc) return rc;
     }
     ctx->points_done += np;
-    if (wmode == 0 || wmode == 3) ctx->norm_sum += (double)np;
+    if (wmode == 0 || wmode == 3 || wmode == 5) ctx->norm_sum += (double)np;
     else if (wmode == 1) ctx->norm_sum += (double)np * scale;
     return 0;
 }
@@ -916,22 +919,20 @@ int scp3g_h_download(scp3g_ctx *ctx, double *M_host, int raw)
     }
     if (ctx->points_done <= 0) return fail(ctx, -6, "no points accumulated");
     std::vector<double> dg = ho_diagonal(ctx);
-    double *d_diag = nullptr, *d_out = nullptr;
-    if ((rc = check_hip(ctx, hipMalloc(&d_diag, n * 8), "alloc diag"))) return rc;
-    if ((rc = check_hip(ctx, hipMalloc(&d_out, bytes), "alloc Mout"))) {
-        hipFree(d_diag);
+    // the staging buffers stay with the running sums (freed with them)
+    if (!ctx->d_diag && (rc = check_hip(ctx, hipMalloc(&ctx->d_diag, n * 8), "alloc diag"))) return rc;
+    if (!ctx->d_mout && (rc = check_hip(ctx, hipMalloc(&ctx->d_mout, bytes), "alloc Mout"))) return rc;
+    if ((rc = check_hip(ctx, hipMemcpyAsync(ctx->d_diag, dg.data(), n * 8, hipMemcpyHostToDevice, ctx->stream),
+                        "upload diag")))
         return rc;
-    }
-    hipMemcpy(d_diag, dg.data(), n * 8, hipMemcpyHostToDevice);
     hipEvent_t ev;
     timer_begin(ctx, "finalize", &ev);
-    rc = check_hip(ctx, launch_finalize(ctx, Hs, n, ctx->norm_sum, d_diag, d_out),
-                   "finalize");
+    rc = check_hip(ctx, launch_finalize(ctx, Hs, n, ctx->norm_sum, ctx->d_diag, ctx->d_mout), "finalize");
     timer_end(ctx, "finalize", ev);
-    if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
-    if (!rc) rc = check_hip(ctx, hipMemcpy(M_host, d_out, bytes, hipMemcpyDeviceToHost), "download Mout");
-    hipFree(d_diag);
-    hipFree(d_out);
+    // dg is read by the queued upload: wait before returning either way
+    const int rs = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (!rc) rc = rs;
+    if (!rc) rc = check_hip(ctx, hipMemcpy(M_host, ctx->d_mout, bytes, hipMemcpyDeviceToHost), "download Mout");
     return rc;
 }
 

@@ -864,6 +864,17 @@ int scp3g_accumulate_surrogate(scp3g_ctx *ctx, int64_t skip, int64_t npts)
     return scp3g_accumulate_mode(ctx, skip, npts, 1, 1.0);
 }
 
+int scp3g_accumulate_correction(scp3g_ctx *ctx, int64_t skip, int64_t npts)
+{
+    int rc = mlp_ready(ctx);
+    if (rc) return rc;
+    if (npts < 0 || skip < 0) return fail(ctx, -4, "negative skip or point count");
+    if (ctx->estimator != SCP3G_EST_ENERGY)
+        return fail(ctx, -4, "the control variate needs the energy (-G) estimator");
+    // weights bV - w_nn, normaliser += npts
+    return scp3g_accumulate_mode(ctx, skip, npts, 5, 1.0);
+}
+
 int scp3g_accumulate_cv(scp3g_ctx *ctx, int64_t skip_nn, int64_t P_nn, int64_t skip_t, int64_t P_t)
 {
     int rc = mlp_ready(ctx);

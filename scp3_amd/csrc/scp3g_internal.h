@@ -115,6 +115,9 @@ struct scp3g_ctx {
     int ngwork = 0, ngfix = 0, grad_form = -1, grad_tile = 64;
     int64_t grad_n = 0;
     int64_t h_n = 0;
+    // Mout staging of scp3g_h_download (n x n) and the diagonal it adds,
+    // kept with the running sums
+    double *d_mout = nullptr, *d_diag = nullptr;
     int64_t points_done = 0;
     // normaliser of the running sum: the number of points for the plain
     // estimator, the sum of per-point scales for the control-variate one

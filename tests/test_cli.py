@@ -99,8 +99,16 @@ def test_scp3_run_and_eigensolver(tmp_path):
         c.accumulate(1 << 30, 1 << 17)
         ev = c.lowest_levels(5)
         e1 = c.lowest_levels_sub(c.sub_basis_size(1), 1)[0]
+        Mout = c.h_download()
     np.testing.assert_allclose(lv[:, 0], ev, rtol=0, atol=1e-12)
     assert abs(e0[-1, 1] - e1 * scp3_amd.AUTOCM) < 1e-6
+    # the E0 monitor (SCP3.cpp:240-247) against LAPACK on the leading blocks
+    # of the same Mout: singles (1 + M) and singles + doubles
+    import scipy.linalg
+    A = np.triu(Mout) + np.triu(Mout, 1).T
+    for col, nsub in ((1, 1 + 12), (2, 1 + 12 + 12 * 13 // 2)):
+        lo = scipy.linalg.eigh(A[:nsub, :nsub], eigvals_only=True, subset_by_index=[0, 0])[0]
+        assert abs(e0[-1, col] - lo * scp3_amd.AUTOCM) < 1e-6, (col, e0[-1, col], lo * scp3_amd.AUTOCM)
     # eigensolver on the FP32 sM file (eigensolver.cpp:83-109 output format)
     r = run(["eigensolver", "sM_0131072.h5", "5"], str(tmp_path))
     assert r.returncode == 0, r.stderr
@@ -143,3 +151,31 @@ def test_scp1_options(tmp_path):
     assert r.returncode != 0 and "too large" in r.stderr
     r = run(["SCP1", str(tmp_path / "missing.dat")], str(tmp_path))
     assert r.returncode != 0 and "File not found" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("modes", ["0-3,7", "7,0-3"])
+def test_scp3_mode_list_skip_block_and_triples(tmp_path, modes):
+    """-m (SCP3.cpp:63-89, the list kept in the given order: -2/-3 take its
+    first entries), -S (log2 skip), -B not a multiple of 64 (blocks of 1000
+    points until a block ends at or past 2^14: 17000 points, normalised by
+    the points summed) and -3, against the binding on the same seeds."""
+    import scp3_amd
+    from scp3_amd import system
+    print("cli case: dimer -m %s -S 20 -B 1000 -2 4 -3 2" % modes, flush=True)
+    r = run(["SCP3", "-G", "-N", "14", "-S", "20", "-B", "1000", "-m", modes, "-2", "4", "-3", "2",
+             "--levels", "5", DIMER], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert "5 modes" in r.stderr
+    lv = np.loadtxt(tmp_path / "levels.dat")
+    sel = np.array([7, 0, 1, 2, 3] if modes.startswith("7") else [0, 1, 2, 3, 7], dtype=np.int32)
+    sysd = system.prepare(DIMER, nmodes2=4, nmodes3=2)
+    with scp3_amd.Context(0) as c:
+        c.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
+        c.set_basis(sel, 4, 2)
+        c.set_pes("qtip4pf")
+        assert c.basis_size() == 1 + 5 + 10 + 4
+        c.reset()
+        c.accumulate(1 << 20, 17000)
+        ev = c.lowest_levels(5)
+    np.testing.assert_allclose(lv[:, 0], ev, rtol=0, atol=1e-12)

@@ -33,11 +33,17 @@ def test_oracle_gradient_is_the_derivative():
 
 
 @pytest.mark.gpu
-def test_device_gradient_matches_oracle(gpu_ctx_factory, hexamer):
-    import scp3_amd
+@pytest.mark.parametrize("name,M,h", [("hexamer_qtip4pf", 48, 256), ("w20_ttm3f", 174, 512)])
+def test_device_gradient_matches_oracle(gpu_ctx_factory, name, M, h):
+    """The device loss gradient against the FP64 oracle at the networks of
+    the hexamer (48-256-256-256-1) and the 20-water cluster (174-512-512-
+    512-1, inputs padded to 192)."""
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path(name), nmodes2=1, nmodes3=0)
+    assert sysd["omega"].size == M
+    print("mlp gradient case %s %d-%d" % (name, M, h), flush=True)
     ctx = gpu_ctx_factory(0)
-    ctx.set_system(hexamer["x0"], hexamer["omega"], hexamer["MUa"])
-    M, h = 48, 256
+    ctx.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
     ctx.mlp_init(h, seed=11)
     P = mlp_oracle.init_params(M, h, 5)
     ctx.mlp_set_params(P, np.array([0.0, 0.0, 1.0]))
@@ -49,17 +55,17 @@ def test_device_gradient_matches_oracle(gpu_ctx_factory, hexamer):
     g_dev, loss_dev = ctx.mlp_grad(B)
     # the oracle sees the same BF16-rounded layer weights the device multiplies with
     Pb = P.copy()
-    nW = 3 * h * 0 + h * 64 + h + 2 * (h * h + h)
     u = Pb.view(np.uint32).copy()
     u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
     rb = u.view(np.float32)
     W, dpad = mlp_oracle.unpack(P, M, h)
+    assert dpad == (M + 63) // 64 * 64
     o = 0
     for l in range(3):
         n_in = dpad if l == 0 else h
         Pb[o:o + h * n_in] = rb[o:o + h * n_in]
         o += h * n_in + h
-    assert o == nW
+    assert o == h * dpad + h + 2 * (h * h + h)
     loss_ref, g_ref = mlp_oracle.loss_grad(Pb.astype(np.float64), y, t.astype(np.float64), M, h)
     rel = np.linalg.norm(g_dev - g_ref) / np.linalg.norm(g_ref)
     cos = g_dev.dot(g_ref) / (np.linalg.norm(g_dev) * np.linalg.norm(g_ref))
@@ -68,6 +74,45 @@ def test_device_gradient_matches_oracle(gpu_ctx_factory, hexamer):
     assert rel < 5e-2 and cos > 0.998
     assert abs(loss_dev - loss_ref) < 2e-2 * abs(loss_ref)
     del Pb, W, dpad
+
+
+@pytest.mark.gpu
+def test_adam_steps_match_restatement(gpu_ctx_factory, hexamer):
+    """Two Adam steps (Kingma & Ba 2015; beta1 0.9, beta2 0.999, eps 1e-8,
+    bias-corrected moments) on the FP32 master parameters, against a numpy
+    FP32 restatement fed the device's own gradients (scp3g_mlp_grad at the
+    same parameters and batch; the training set is one batch, so both steps
+    see it).  The first step moves every parameter with a gradient well
+    above eps by lr: the bias correction makes m/sqrt(v) = sign(g)."""
+    ctx = gpu_ctx_factory(0)
+    ctx.set_system(hexamer["x0"], hexamer["omega"], hexamer["MUa"])
+    M, h, B, lr = 48, 64, 256, 1e-3
+    ctx.mlp_init(h, seed=4)
+    P0 = mlp_oracle.init_params(M, h, 9)
+    ctx.mlp_set_params(P0, np.array([0.0, 0.0, 1.0]))
+    rng = np.random.default_rng(8)
+    ctx.mlp_load_data(rng.normal(scale=np.sqrt(0.5), size=(B, M)), rng.normal(size=B).astype(np.float32))
+    f32 = np.float32
+    b1, b2, eps = f32(0.9), f32(0.999), f32(1e-8)
+    m = np.zeros_like(P0)
+    v = np.zeros_like(P0)
+    P = ctx.mlp_params()[0]
+    np.testing.assert_array_equal(P, P0)
+    for step in (1, 2):
+        g, _ = ctx.mlp_grad(B)
+        ctx.mlp_train(1, B, lr)
+        P_dev = ctx.mlp_params()[0]
+        m = b1 * m + (f32(1) - b1) * g
+        v = b2 * v + (f32(1) - b2) * g * g
+        c1 = f32(1) - b1 ** step
+        c2 = f32(1) - b2 ** step
+        P = P - f32(lr) * (m / c1) / (np.sqrt(v / c2) + eps)
+        np.testing.assert_allclose(P_dev, P, rtol=0, atol=1e-3 * lr + 4e-7 * np.abs(P).max())
+        if step == 1:
+            big = np.abs(g) > 1e-4
+            assert big.sum() > 1000
+            np.testing.assert_allclose(np.abs(P_dev - P0)[big], lr, rtol=2e-2)
+        P = P_dev.copy()
 
 
 @pytest.mark.gpu
@@ -193,3 +238,37 @@ def test_surrogate_term_split_bf16_products(gpu_ctx_factory, hexamer, terms):
     else:
         assert err64 < 1e-5 * np.abs(H64).max()
         assert err64 < 0.02 * model
+
+
+@pytest.mark.gpu
+def test_control_variate_terms_separately(gpu_ctx_factory, hexamer):
+    """scp3g_accumulate_surrogate and scp3g_accumulate_correction, each in a
+    sum of its own, add up to scp3g_accumulate_cv on the same seeds:
+    H_cv = S / P_nn + C / P_t (FP64, raw sums)."""
+    import scp3_amd
+    ctx = gpu_ctx_factory(0)
+    _small_basis(ctx, hexamer, scp3_amd.FP64)
+    ctx.mlp_init(64, seed=5)
+    ctx.mlp_sample_data(1 << 40, 4096)
+    ctx.mlp_train(50, 1024, 1e-3)
+    skip_nn, Pnn, skip_t, Pt = 1 << 30, 2 * 4096 + 300, (1 << 30) + 77, 4096 + 11
+    ctx.reset()
+    ctx.accumulate_surrogate(skip_nn, Pnn)
+    S = ctx.h_download(raw=True)
+    ctx.reset()
+    ctx.accumulate_correction(skip_t, Pt)
+    C = ctx.h_download(raw=True)
+    assert ctx.points_done() == Pt
+    ctx.reset()
+    ctx.accumulate_cv(skip_nn, Pnn, skip_t, Pt)
+    H = ctx.h_download(raw=True)
+    ref = S / Pnn + C / Pt
+    assert np.abs(H - ref).max() < 1e-12 * np.abs(ref).max()
+    # each term's Mout is its mean plus the same HO diagonal
+    ctx.reset()
+    ctx.accumulate_correction(skip_t, Pt)
+    Mc = ctx.h_download()
+    ctx.reset()
+    ctx.accumulate_surrogate(skip_nn, Pnn)
+    Ms = ctx.h_download()
+    np.testing.assert_allclose(Mc - C / Pt, Ms - S / Pnn, rtol=0, atol=1e-12 * np.abs(ref).max())

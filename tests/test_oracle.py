@@ -342,3 +342,24 @@ def test_harmonic_limit_of_the_map(hexamer):
         dV = (oracle.qtip4pf(r) - V0) / AUTOKCALPMOL
         harm = 0.5 * np.sum(hexamer["omega"] * y * y)
         assert abs(dV - harm) < 5e-3 * harm
+
+
+@pytest.mark.parametrize("name,pes", [("dimer_qtip4pf", "qtip4pf"), ("hexamer_ttm3f", "ttm3f")])
+def test_cpu_baseline_blocks(name, pes):
+    """The CPU baseline's block pieces (serial potential loop through a C
+    function pointer, OpenMP get_bra) equal the oracle's block loop on the
+    same scrambled points, with the reference build's potential and with the
+    oracle's restatement."""
+    from scp3_amd import system
+    sysd = system.prepare(system.input_path(name), nmodes2=6, nmodes3=3)
+    M0 = sysd["omega"].size
+    y = oracle.sobol_normals(M0, 1 << 30, 300, scramble=0)
+    o = oracle.scp3_accumulate(sysd, pes, 0, 300, normals=y * np.sqrt(2.0), want=("bV",))
+    for reference in (True, False):
+        fp, kind = oracle.pes_pointer(pes, reference=reference)
+        if reference and kind != "reference":
+            continue
+        bV = oracle.block_weights(sysd, fp, y)
+        np.testing.assert_allclose(bV, o["bV"], rtol=1e-12, atol=1e-15)
+    b = oracle.block_bra(sysd, y)
+    np.testing.assert_array_equal(b, oracle.get_bra(y[:, sysd["modes"]], M0, 6, 3))

@@ -59,6 +59,20 @@ def test_inputs_are_minima():
         assert np.abs(g).max() < 1e-4
 
 
+def test_ttm3f_inputs_are_minima():
+    """The TTM3-F inputs (hexamer, 20-water cluster) sit at a minimum of
+    TTM3-F: the oracle's analytic gradient vanishes there (kcal/mol/A) and
+    every kept frequency of the input Hessian is real and positive."""
+    from oracle import oracle
+    for name, m in (("hexamer_ttm3f", 48), ("w20_ttm3f", 174)):
+        sp, x, H = system.read_vladimir(system.input_path(name))
+        _, xs, _, _ = system.ohhohh(sp, x)
+        E, g = oracle.ttm3f_grad(xs)
+        assert np.abs(g).max() < 1e-4, (name, np.abs(g).max())
+        sysd = system.prepare(system.input_path(name), nmodes2=1)
+        assert sysd["omega"].size == m and sysd["omega"][0] > 10.0 / scp3_amd.AUTOCM
+
+
 def test_normals_file_round_trip(tmp_path):
     from scp3_amd import system
     rng = np.random.default_rng(4)

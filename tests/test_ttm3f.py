@@ -103,8 +103,13 @@ def test_gpu_ttm3f_matches_oracle(gpu_ctx_factory, name, nw):
         Eg = E.cpu().numpy()
         if fp32:
             err = np.abs(Eg - ref).max()
-            print("ttm3f fp32 max |dE| = %.3e kcal/mol (%s)" % (err, name))
+            bias = (Eg - ref).mean()
+            print("ttm3f fp32 max |dE| = %.3e, mean %.2e kcal/mol (%s)" % (err, bias, name))
+            # per configuration 5e-3 kcal/mol (1.7 cm^-1); the mean over the
+            # configurations, which is what shifts the matrix elements, within
+            # 5e-4 kcal/mol (0.17 cm^-1)
             assert err < 5e-3
+            assert abs(bias) < 5e-4
         else:
             np.testing.assert_allclose(Eg, ref, rtol=1e-12, atol=1e-11)
 
