@@ -241,8 +241,10 @@ __global__ void k_to_bf16(const float *__restrict__ W, int rows, int cols,
     WbT[(size_t)c * rows + r] = h;
 }
 
-// w_nn = V0 + t_mean + t_std * f  and, when V (hartree) is given, the
-// correction weight w - w_nn = V - V_nn (SCP3.cpp:222-223).
+// mode 0: the surrogate weight w_nn = V0 + t_mean + t_std * f; mode 1: the
+// correction weight w - w_nn = V - V_nn from the energies E (kcal/mol,
+// SCP3.cpp:222-223); mode 2: the surrogate potential V_nn = w_nn + 1/2
+// sum_k omega_k y_k^2 (hartree).
 __global__ void k_mlp_weights(const float *__restrict__ f, const double *__restrict__ y,
                               int64_t ldy, const double *__restrict__ omega, int M, double base,
                               double tstd, int64_t npts, int64_t npad, const double *__restrict__ E,
@@ -263,6 +265,10 @@ __global__ void k_mlp_weights(const float *__restrict__ f, const double *__restr
     for (int k = 0; k < M; ++k) {
         const double yk = y[(size_t)k * ldy + i];
         hsum += yk * (omega[k] * yk);
+    }
+    if (mode == 2) {
+        w[i] = scale * (wnn + 0.5 * hsum);
+        return;
     }
     const double bV = E[i] / kAutokcalpmol - 0.5 * hsum;
     w[i] = scale * (bV - wnn);
@@ -840,10 +846,10 @@ int scp3g_mlp_eval(scp3g_ctx *ctx, const double *y_dev, int64_t ldy, int64_t npt
         const int64_t np = std::min<int64_t>(T, npts - done);
         hipError_t e = eval_f(ctx, s, y_dev + done, ldy, np);
         if (e == hipSuccess) {
-            // V_nn = w_nn + 1/2 sum omega y^2: mode 0 gives w_nn
+            // V_nn = w_nn + 1/2 sum omega y^2 (mode 2)
             hipLaunchKernelGGL(k_mlp_weights, dim3((unsigned)((np + 255) / 256)), dim3(256), 0,
                                ctx->stream, s->out, y_dev + done, ldy, ctx->d_omega, s->M,
-                               s->V0 + s->t_mean, s->t_std, np, np, (const double *)nullptr, 0, 1.0,
+                               s->V0 + s->t_mean, s->t_std, np, np, (const double *)nullptr, 2, 1.0,
                                V_dev + done);
             e = hipGetLastError();
         }
