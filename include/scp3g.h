@@ -264,6 +264,18 @@ int scp3g_accumulate_correction(scp3g_ctx *ctx, int64_t skip, int64_t npts);
  */
 int scp3g_sobol_normals(scp3g_ctx *ctx, int dims, int64_t skip, int64_t npts, int halve,
                         double *y_dev, int64_t ldy, uint64_t *ints_dev);
+/*
+ * The same points as scp3g_sobol_normals, from an entry point that
+ * allocates nothing: no counterpart in the reference, for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops).  The direction integers are staged in a fixed device array of the
+ * library (1111 x 62 words, 551 KiB per GPU, held until the process ends)
+ * instead of a buffer allocated per call; dims above 1111 are rejected.
+ * Stream-ordered on the context's stream and does not wait for it.  Calls
+ * that overlap on one GPU (several contexts or threads) race on that array.
+ */
+int scp3g_sobol_normals_noalloc(scp3g_ctx *ctx, int dims, int64_t skip, int64_t npts, int halve,
+                                double *y_dev, int64_t ldy, uint64_t *ints_dev);
 
 /*
  * crd_dev[c*ldc + i] = bohr*(x0[c] + sum_k MUa[c,k] y[k*ldy + i]) for

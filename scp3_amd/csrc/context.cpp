@@ -305,6 +305,12 @@ int scp3g_create(scp3g_ctx **out, int device)
         delete ctx;
         return -3;
     }
+    // the static arrays of scp3g_sobol_normals_noalloc, once per device
+    if (sobol_static_setup(device) != hipSuccess) {
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return -3;
+    }
     *out = ctx;
     return 0;
 }

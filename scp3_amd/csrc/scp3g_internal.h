@@ -237,6 +237,8 @@ void release_davidson(scp3g_ctx *ctx);
 // host helpers
 std::vector<double> ho_diagonal(const scp3g_ctx *ctx);  // HO diagonal + unselected ZPE
 void sobol_directions(int dims, uint64_t *v);  // dims x 62, scaled
+// device addresses of the allocation-free sampler's arrays, once per device
+hipError_t sobol_static_setup(int device);
 // affine digit scrambling of the direction integers in place, shifts dims
 void sobol_scramble(int dims, uint64_t seed, uint64_t *v, uint64_t *shift);
 // lowest nvec eigenpairs of a small dense symmetric matrix (column-major,

@@ -15,6 +15,8 @@
 // the device log() used in BSM's tail branch.
 #include "scp3g_internal.h"
 
+#include <mutex>
+
 #include "sobol_dirnums.inc"
 
 namespace scp3g {
@@ -171,4 +173,99 @@ hipError_t launch_sobol(scp3g_ctx *ctx, const uint64_t *dirs, int dims, int64_t 
     return hipGetLastError();
 }
 
+// ---- the allocation-free sampler (scp3g_sobol_normals_noalloc) -----------
+//
+// scp3g_sobol_normals stages the direction integers of its call in a buffer
+// it allocates; callers that cannot allocate (graph capture, serving loops)
+// get the same kernels with that staging in a fixed device array instead.
+// `scratch` holds the direction integers of the call's dimensions (at most
+// the table's 1111 x 62 words, 551 KiB per GPU, kept until the process
+// ends); a kernel rebuilds them from the Bratley-Fox base table on every
+// call, so nothing in it needs clearing.  Calls that overlap on one GPU
+// race on it.
+constexpr int kBaseWords = 1 + SCP3_SOBOL_INIT_MAX;  // polynomial, initial numbers
+constexpr int kMaxDevices = 64;
+static __device__ unsigned short sobol_base[SCP3_SOBOL_DIM_MAX * kBaseWords];
+static __device__ __align__(256) uint64_t scratch[SCP3_SOBOL_DIM_MAX * kSobolCols];
+
+struct StaticTables {
+    bool ready = false;
+    uint64_t *scratch = nullptr;      // device address of `scratch`
+    unsigned short *base = nullptr;   // device address of `sobol_base`
+};
+static StaticTables g_static[kMaxDevices];
+static std::mutex g_static_lock;
+
+// Once per device, with the device current: the device addresses of the two
+// arrays (the host names are stand-ins, not device addresses) and the base
+// table's contents.
+hipError_t sobol_static_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(g_static_lock);
+    StaticTables &t = g_static[device];
+    if (t.ready) return hipSuccess;
+    void *ps = nullptr, *pb = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ps, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if ((e = hipGetSymbolAddress(&pb, HIP_SYMBOL(sobol_base))) != hipSuccess) return e;
+    std::vector<unsigned short> base((size_t)SCP3_SOBOL_DIM_MAX * kBaseWords, 0);
+    for (int d = 0; d < SCP3_SOBOL_DIM_MAX; ++d) {
+        base[(size_t)d * kBaseWords] = scp3_sobol_poly[d];
+        for (int j = 0; j < SCP3_SOBOL_INIT_MAX; ++j)
+            base[(size_t)d * kBaseWords + 1 + j] = scp3_sobol_init[d][j];
+    }
+    if ((e = hipMemcpy(pb, base.data(), base.size() * sizeof(unsigned short), hipMemcpyHostToDevice)) != hipSuccess)
+        return e;
+    t.scratch = static_cast<uint64_t *>(ps);
+    t.base = static_cast<unsigned short *>(pb);
+    t.ready = true;
+    return hipSuccess;
+}
+
+// sobol_directions on the device: one lane per dimension runs the
+// Bratley-Fox recurrence (sobol.cpp:27700-27780) in its own row of v and
+// then scales column j by 2^(61 - j).
+__global__ void __launch_bounds__(64)
+    k_sobol_dirs(const unsigned short *__restrict__ base, int dims, uint64_t *__restrict__ v)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= dims) return;
+    uint64_t *row = v + (size_t)d * kSobolCols;
+    if (d == 0) {
+        for (int j = 0; j < kSobolCols; j++) row[j] = 1;
+    } else {
+        const unsigned p = base[(size_t)d * kBaseWords];
+        const int m = 31 - __builtin_clz(p);
+        for (int j = 0; j < m; j++) row[j] = base[(size_t)d * kBaseWords + 1 + j];
+        for (int j = m; j < kSobolCols; j++) {
+            uint64_t nv = row[j - m];
+            for (int k = 0; k < m; k++)
+                if ((p >> (m - 1 - k)) & 1u) nv ^= row[j - k - 1] << (k + 1);
+            row[j] = nv;
+        }
+    }
+    for (int j = 0; j < kSobolCols; j++) row[j] <<= (kSobolCols - 1 - j);
+}
+
 }  // namespace scp3g
+
+extern "C" int scp3g_sobol_normals_noalloc(scp3g_ctx *ctx, int dims, int64_t skip, int64_t npts, int halve,
+                                           double *y_dev, int64_t ldy, uint64_t *ints_dev)
+{
+    using namespace scp3g;
+    if (!ctx) return -1;
+    if (dims < 1 || dims > SCP3_SOBOL_DIM_MAX)
+        return fail(ctx, -4, "dims must be in 1..1111 (the static direction table holds 1111 dimensions)");
+    if (npts < 0 || skip < 0 || ldy < npts) return fail(ctx, -4, "bad sizes");
+    if (skip + npts + 64 * 16 >= ((int64_t)1 << 60)) return fail(ctx, -4, "SEED too large");
+    if (!y_dev) return fail(ctx, -4, "null output");
+    const StaticTables &t = g_static[ctx->device];
+    if (!t.ready) return fail(ctx, -2, "static Sobol tables not set up on this device");
+    hipSetDevice(ctx->device);
+    hipLaunchKernelGGL(k_sobol_dirs, dim3((unsigned)((dims + 63) / 64)), dim3(64), 0, ctx->stream, t.base, dims,
+                       t.scratch);
+    int rc = check_hip(ctx, hipGetLastError(), "sobol directions");
+    if (rc) return rc;
+    return check_hip(ctx, launch_sobol(ctx, t.scratch, dims, skip, npts, halve, y_dev, ldy, ints_dev), "sobol");
+}
