@@ -56,10 +56,11 @@ PRESETS = {
     "c1": dict(config="dimer_qtip4pf", n2=12, n3=0, points=16, tile=0, levels=5, forms="fp64,mixed"),
     "c2": dict(config="hexamer_qtip4pf", n2=48, n3=0, points=20, tile=0, levels=10, forms="fp64,mixed"),
     # the default: the plain FP64 and mixed forms over 2^21 points per step
-    # (FP64 capped at 2 timed steps, 15 s each), and the control variate as
-    # c4 runs it (2^24 surrogate + 2^21 true points, one timed step)
+    # (FP64 capped at 2 timed steps, 15 s each), and the control variate with
+    # its correction term over the 2^23 true points that 0.5 cm^-1 needs
+    # (2^24 surrogate points, one timed step)
     "c3": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=0, levels=10, forms="mixed,fp64,cv",
-               cv_nn=24, cv_t=21, mlp_hidden=256, mlp_steps=20000, mlp_batch=4096, mlp_pairs=18,
+               cv_nn=24, cv_t=23, mlp_hidden=256, mlp_steps=20000, mlp_batch=4096, mlp_pairs=18,
                caps={"fp64": (2, 1), "cv": (1, 1)}),
     "c4": dict(config="hexamer_ttm3f", n2=48, n3=48, points=21, tile=0, levels=10, forms="fp64,cv",
                cv_nn=24, cv_t=19, mlp_hidden=256, mlp_steps=20000, mlp_batch=4096, mlp_pairs=18),
@@ -182,15 +183,20 @@ def converge(a, pr, sysd):
                 Ct = c.h_download(raw=True) / Pt
                 c.reset()
             for P in cps:
-                c.sync()
-                t0 = time.perf_counter()
-                if form == "cv":
-                    c.accumulate_surrogate(skip + done, P - done)
-                else:
-                    c.accumulate(skip + done, P - done)
-                c.sync()
-                t_acc += time.perf_counter() - t0
-                done = P
+                # 2^25-point pieces, each with a progress line on stderr
+                while done < P:
+                    m = min(1 << 25, P - done)
+                    c.sync()
+                    t0 = time.perf_counter()
+                    if form == "cv":
+                        c.accumulate_surrogate(skip + done, m)
+                    else:
+                        c.accumulate(skip + done, m)
+                    c.sync()
+                    t_acc += time.perf_counter() - t0
+                    done += m
+                    print("converge: %s seed %d, %d points, %.1f s" % (form, seed, done, t_acc),
+                          file=sys.stderr, flush=True)
                 t1 = time.perf_counter()
                 if form == "cv":
                     M = c.h_download()
@@ -207,7 +213,7 @@ def converge(a, pr, sysd):
                                   "levels_cm1": [round(float(v) * scp3_amd.AUTOCM, 4) for v in ev]}), flush=True)
 
 
-def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
+def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra, keep_h=False):
     import scp3_amd
     mode = scp3_amd.FP64 if form in ("fp64", "grad", "stein") else scp3_amd.MIXED
     setup(c, sysd, pr, mode, form)
@@ -259,7 +265,31 @@ def run_form(c, sysd, pr, form, P, skip, steps, warmup, k, extra):
         t_lev += time.perf_counter() - t1
     npts = c.points_done()
     return dict(t_acc=t_acc / steps, t_lev=t_lev / steps, levels=ev, kernels=kt, n=c.basis_size(),
-                points=npts, train_s=train_s)
+                points=npts, train_s=train_s, Mout=c.h_download() if keep_h else None)
+
+
+def error_budget(Ma, Mb, la, lb):
+    """Max-abs and Frobenius norm of dH = Ma - Mb over the upper triangles
+    (absolute in hartree and relative to Mb's), and each level's difference."""
+    import scp3_amd
+    out = {}
+    # column blocks keep the temporaries small at 20825 states
+    mx = fro2 = ref_mx = ref_fro2 = 0.0
+    n = Ma.shape[0]
+    for c0 in range(0, n, 2048):
+        A = np.triu(Ma[:, c0:c0 + 2048], -c0)
+        B = np.triu(Mb[:, c0:c0 + 2048], -c0)
+        D = A - B
+        mx = max(mx, float(np.abs(D).max()))
+        fro2 += float((D * D).sum())
+        ref_mx = max(ref_mx, float(np.abs(B).max()))
+        ref_fro2 += float((B * B).sum())
+    out["dH_max_abs_hartree"] = mx
+    out["dH_max_abs_rel"] = mx / ref_mx
+    out["dH_frobenius_hartree"] = float(np.sqrt(fro2))
+    out["dH_frobenius_rel"] = float(np.sqrt(fro2 / ref_fro2))
+    out["level_diff_cm1"] = [round(float(v), 5) for v in (np.asarray(la) - np.asarray(lb)) * scp3_amd.AUTOCM]
+    return out
 
 
 def pmc_traffic(kernel, config, n, points_per_launch):
@@ -428,13 +458,16 @@ def write_convergence(pr, files, n):
             dev = float(np.abs(np.array(r["levels_cm1"]) - ref).max())
             table.append({"points": r["points"], "points_t": r.get("points_t"), "seed": r["seed"],
                           "accumulate_s": r["accumulate_s"], "train_s": r.get("train_s", 0.0),
-                          "max_dev_cm1": round(dev, 3)})
+                          "max_dev_cm1": round(dev, 3),
+                          "dev_cm1": [round(float(v), 3) for v in np.array(r["levels_cm1"]) - ref]})
         req = None
         for q in range(len(table)):
             if all(t["max_dev_cm1"] <= 0.5 for t in table[q:]):
                 req = table[q]["points"]
                 break
-        rec["forms"][form] = {"checkpoints": table, "points_required": req}
+        rec["forms"][form] = {"checkpoints": table, "points_required": req,
+                              "points_t": table[0]["points_t"] if table else None,
+                              "dev_at_required_cm1": next((t["dev_cm1"] for t in table if t["points"] == req), None)}
     path = convergence_path(pr, n)
     with open(path, "w") as f:
         json.dump(rec, f, indent=1)
@@ -458,7 +491,8 @@ def time_to_half_wavenumber(pr, res, n):
         if f in res and plain:
             rr = res[f]
             rate = rr["points"] / rr["t_acc"]
-            out[f] = {"points": plain, "points_per_s": rate, "s": plain / rate + rr["t_lev"]}
+            out[f] = {"points": plain, "points_per_s": rate, "s": plain / rate + rr["t_lev"],
+                      "level_error_cm1": rec["forms"]["mixed"]["dev_at_required_cm1"]}
     cvr = rec["forms"].get("cv", {}).get("points_required")
     if "cv" in res and cvr and "mixed" in res:
         rr = res["cv"]
@@ -466,7 +500,10 @@ def time_to_half_wavenumber(pr, res, n):
         t_t = Pt / (res["mixed"]["points"] / res["mixed"]["t_acc"])
         rate_nn = (1 << pr["cv_nn"]) / max(rr["t_acc"] - t_t, 1e-9)
         out["cv"] = {"points": cvr, "points_t": Pt, "surrogate_points_per_s": rate_nn,
-                     "train_s": rr["train_s"], "s": rr["train_s"] + cvr / rate_nn + t_t + rr["t_lev"]}
+                     "train_s": rr["train_s"], "s": rr["train_s"] + cvr / rate_nn + t_t + rr["t_lev"],
+                     "level_error_cm1": rec["forms"]["cv"]["dev_at_required_cm1"]}
+        if rec["forms"]["cv"].get("points_t") not in (None, Pt):
+            out["cv"]["note"] = "the record's correction term had %d true points" % rec["forms"]["cv"]["points_t"]
     if "fp64" in out and "cv" in out:
         out["speedup_cv_vs_fp64"] = out["fp64"]["s"] / out["cv"]["s"]
     if "fp64" in out and "mixed" in out:
@@ -524,7 +561,8 @@ def main():
             nsteps[f] = (st, wu)
             print("bench: preset %s form %s (%d steps, %d warm-up)" % (a.preset, f, st, wu),
                   file=sys.stderr, flush=True)
-            res[f] = run_form(c, sysd, pr, f, P, skip, st, wu, pr["levels"], extra)
+            res[f] = run_form(c, sysd, pr, f, P, skip, st, wu, pr["levels"], extra,
+                              keep_h=f in ("fp64", "mixed") and "fp64" in forms and "mixed" in forms)
             print("bench: form %s accumulate %.4f s, levels %.4f s" % (f, res[f]["t_acc"], res[f]["t_lev"]),
                   file=sys.stderr, flush=True)
     n = res[forms[0]]["n"]
@@ -590,6 +628,12 @@ def main():
     if "fp64" in res and main_form != "fp64":
         d = np.abs(res[main_form]["levels"] - res["fp64"]["levels"]).max() * scp3_amd.AUTOCM
         out["max_level_diff_vs_fp64_cm1"] = float(d)
+    if "fp64" in res and "mixed" in res and res["mixed"]["Mout"] is not None:
+        # the mixed form against the FP64 form on the same points
+        out["error_budget_mixed_vs_fp64"] = error_budget(res["mixed"]["Mout"], res["fp64"]["Mout"],
+                                                         res["mixed"]["levels"], res["fp64"]["levels"])
+    for rr in res.values():
+        rr.pop("Mout", None)
     tt = time_to_half_wavenumber(pr, res, n)
     if tt:
         out["time_to_0p5_cm1"] = tt
