@@ -100,6 +100,7 @@ def parse():
     ap.add_argument("--seeds", default="0", help="scramble seeds of the convergence study")
     ap.add_argument("--level-tol", type=float, default=1e-7, help="Davidson residual tolerance (hartree)")
     ap.add_argument("--cv-t", type=int, default=None, help="log2 of true-potential points of the control variate")
+    ap.add_argument("--cv-nn", type=int, default=None, help="log2 of surrogate points of the control variate")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     return ap.parse_args()
@@ -539,6 +540,8 @@ def main():
     sysd["name"] = pr["config"]
     if a.cv_t is not None:
         pr["cv_t"] = a.cv_t
+    if a.cv_nn is not None:
+        pr["cv_nn"] = a.cv_nn
     if a.converge_summary:
         from oracle import oracle
         n = oracle.basis_size(sysd["modes"].size, sysd["nmodes2"], sysd["nmodes3"])
