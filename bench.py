@@ -101,6 +101,7 @@ def parse():
     ap.add_argument("--level-tol", type=float, default=1e-7, help="Davidson residual tolerance (hartree)")
     ap.add_argument("--cv-t", type=int, default=None, help="log2 of true-potential points of the control variate")
     ap.add_argument("--cv-nn", type=int, default=None, help="log2 of surrogate points of the control variate")
+    ap.add_argument("--mlp-steps", type=int, default=None, help="Adam steps of the surrogate's training")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     return ap.parse_args()
@@ -542,6 +543,8 @@ def main():
         pr["cv_t"] = a.cv_t
     if a.cv_nn is not None:
         pr["cv_nn"] = a.cv_nn
+    if a.mlp_steps is not None:
+        pr["mlp_steps"] = a.mlp_steps
     if a.converge_summary:
         from oracle import oracle
         n = oracle.basis_size(sysd["modes"].size, sysd["nmodes2"], sysd["nmodes3"])
