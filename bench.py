@@ -438,10 +438,17 @@ def write_convergence(pr, files, n):
     stein_mixed runs, with the standard error of that mean) and, for each
     -G form, every checkpoint's largest level deviation from them and the
     first checkpoint from which all levels stay within 0.5 cm^-1."""
-    rows = []
+    rows, seen = [], set()
     for fn in files:
-        rows += [json.loads(l) for l in open(fn) if l.strip().startswith("{")]
-    rows = [r for r in rows if "points" in r]
+        for line in open(fn):
+            if not line.strip().startswith("{"):
+                continue
+            r = json.loads(line)
+            key = (r.get("form"), r.get("seed"), r.get("points"), r.get("points_t"))
+            # a run continued from an earlier one repeats that run's last checkpoint
+            if "points" in r and key not in seen:
+                seen.add(key)
+                rows.append(r)
     st = [r for r in rows if r["form"] == "stein_mixed"]
     pmax = max(r["points"] for r in st)
     L = np.array([r["levels_cm1"] for r in st if r["points"] == pmax])
