@@ -41,13 +41,17 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kFT = 256;  // workgroup tile
 
-__device__ __forceinline__ uint4 xor4(uint4 v, uint4 f)
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// An MFMA operand fragment read from LDS with the weight signs of its points
+// applied (sign-bit XOR), as one vector load: a uint4 read is split into
+// four i32 loads of which only the first keeps the LDS variable's alias
+// scope, so the re-merged ds_read_b128 then looks as if it could alias the
+// operand DMA in flight and the compiler waits for that DMA first.
+__device__ __forceinline__ bf16x8 lds_frag_xor(const uint4 &src, u32x4 f)
 {
-    v.x ^= f.x;
-    v.y ^= f.y;
-    v.z ^= f.z;
-    v.w ^= f.w;
-    return v;
+    const u32x4 v = *reinterpret_cast<const u32x4 *>(&src);
+    return __builtin_bit_cast(bf16x8, v ^ f);
 }
 constexpr int kMaxSignWords = 4096;  // 32-point sign words per split (131072 points)
 
@@ -670,17 +674,22 @@ __global__ void __launch_bounds__(512, 1)
     // chunk (l & 3) ^ ((row >> 1) & 3) of that row; so each lane fetches that
     // chunk, and a block is still one contiguous KiB of global memory.
     constexpr int KS = TERMS == 3 ? 1 : 2;  // k-blocks per stage
+    // Row r = rb 16 + (lane >> 2) of a block has (r >> 1) & 3 = (lane >> 3) & 3,
+    // so the lane's chunk offset within a block is the same for all 8 blocks
+    // and the rest of each source address and LDS position is wave-uniform
+    // (scalar registers: 8 per-lane 64-bit pointers would crowd the VGPRs
+    // that hold the accumulators).
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int lane_off = (lane >> 2) * 4 + ((lane & 3) ^ ((lane >> 3) & 3));
     const uint4 *gsrc[8];
     int loff[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const int blk = wave * 8 + i;
+        const int blk = wv * 8 + i;
         const int slot = blk >> 4;
         const int rb = blk & 15;
-        const int r = rb * 16 + (lane >> 2);
-        const int c = (lane & 3) ^ ((r >> 1) & 3);
-        const size_t row = (slot < 2 ? rowA : rowB) + r;
-        gsrc[i] = TERMS == 3 ? ((slot & 1) ? Xl : Xh) + row * 4 + c : Xh + row * 4 + c;
+        const size_t row0 = (slot < 2 ? rowA : rowB) + (size_t)rb * 16;
+        gsrc[i] = (TERMS == 3 ? ((slot & 1) ? Xl : Xh) : Xh) + row0 * 4;
         loff[i] = slot * (kFT * 4) + rb * 64;
     }
     // DMA of the stage that starts at k-block kbv (the second k-block of a
@@ -692,9 +701,9 @@ __global__ void __launch_bounds__(512, 1)
         const int k1_ = k0_ + 1 < kb1 ? k0_ + 1 : k0_;                                            \
         /* TERMS = 1: this wave's 8 blocks all lie in slot (wave >> 1); the odd  */              \
         /* slots hold the stage's second k-block                                 */              \
-        const int kk_ = (TERMS == 1 && ((wave >> 1) & 1)) ? k1_ : k0_;                            \
+        const int kk_ = (TERMS == 1 && ((wv >> 1) & 1)) ? k1_ : k0_;                              \
         _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                           \
-            __builtin_amdgcn_global_load_lds((const void *)(gsrc[i] + (size_t)kk_ * planeStride), \
+            __builtin_amdgcn_global_load_lds((const void *)(gsrc[i] + (size_t)kk_ * planeStride + lane_off), \
                 (__attribute__((address_space(3))) void *)(&(arr)[0][0] + loff[i]), 16, 0, 0);    \
         }                                                                                         \
     } while (0)
@@ -706,25 +715,23 @@ __global__ void __launch_bounds__(512, 1)
 #define X3_KBLOCK(arr, kbv, sa, sb)                                                               \
     do {                                                                                          \
         const unsigned bits_ = (sgn[kbv] >> (8 * lg)) & 0xffu;                                    \
-        uint4 fl_;                                                                                \
+        u32x4 fl_;                                                                                \
         fl_.x = ((bits_ & 1u) << 15) | (((bits_ >> 1) & 1u) << 31);                               \
         fl_.y = (((bits_ >> 2) & 1u) << 15) | (((bits_ >> 3) & 1u) << 31);                        \
         fl_.z = (((bits_ >> 4) & 1u) << 15) | (((bits_ >> 5) & 1u) << 31);                        \
         fl_.w = (((bits_ >> 6) & 1u) << 15) | (((bits_ >> 7) & 1u) << 31);                        \
         bf16x8 ah_[8], al_[8];                                                                    \
         _Pragma("unroll") for (int a = 0; a < 8; ++a) {                                           \
-            const int r_ = wr * 128 + 16 * a + li;                                                \
-            const int p_ = r_ * 4 + (lg ^ ((r_ >> 1) & 3));                                       \
+            const int p_ = pA + 64 * a;                                                           \
             ah_[a] = __builtin_bit_cast(bf16x8, (arr)[sa][p_]);                                   \
             if (TERMS == 3) al_[a] = __builtin_bit_cast(bf16x8, (arr)[(sa) + 1][p_]);             \
         }                                                                                         \
         _Pragma("unroll") for (int half = 0; half < 2; ++half) {                                  \
             bf16x8 bh_[2], bl_[2];                                                                \
             _Pragma("unroll") for (int bb = 0; bb < 2; ++bb) {                                    \
-                const int r_ = wc * 64 + 16 * (2 * half + bb) + li;                               \
-                const int p_ = r_ * 4 + (lg ^ ((r_ >> 1) & 3));                                   \
-                bh_[bb] = __builtin_bit_cast(bf16x8, xor4((arr)[sb][p_], fl_));                   \
-                if (TERMS == 3) bl_[bb] = __builtin_bit_cast(bf16x8, xor4((arr)[(sb) + 1][p_], fl_)); \
+                const int p_ = pB + 64 * (2 * half + bb);                                         \
+                bh_[bb] = lds_frag_xor((arr)[sb][p_], fl_);                                       \
+                if (TERMS == 3) bl_[bb] = lds_frag_xor((arr)[(sb) + 1][p_], fl_);                 \
             }                                                                                     \
             _Pragma("unroll") for (int term = 0; term < TERMS; ++term)                            \
                 _Pragma("unroll") for (int a = 0; a < 8; ++a)                                     \
@@ -736,6 +743,17 @@ __global__ void __launch_bounds__(512, 1)
                     }                                                                             \
         }                                                                                         \
     } while (0)
+    // LDS positions of this lane's fragment rows: row r = wr 128 + 16 a + li
+    // (A) or wc 64 + 16 b + li (B) holds chunk lg at r 4 + (lg ^ ((r >> 1) & 3)),
+    // and (r >> 1) & 3 = (li >> 1) & 3 for every a and b, so fragment a sits at
+    // pA + 64 a.  The empty asm makes pA and pB opaque: the optimiser would
+    // split each address into a chain of getelementptrs deeper than the LDS
+    // lowering pass follows when it tags the reads with their array's alias
+    // scope, and an untagged read waits for the operand DMA in flight.
+    int pA = (wr * 128 + li) * 4 + (lg ^ ((li >> 1) & 3));
+    int pB = (wc * 64 + li) * 4 + (lg ^ ((li >> 1) & 3));
+    asm("" : "+v"(pA));
+    asm("" : "+v"(pB));
     // the k-blocks kbv.. of stage arr (cnt_ of them, at most KS)
 #define X3_STEP(arr, kbv, cnt_)                                                                   \
     do {                                                                                          \
@@ -750,14 +768,19 @@ __global__ void __launch_bounds__(512, 1)
     // read (the compiler's vmcnt(0) before s_barrier) and, through the
     // barrier, for every wave's DMA into it and every wave's reads of the
     // other stage, which the next DMA then overwrites
+    // The sched_barrier after each DMA keeps its issue at the top of the
+    // stage: left alone, the scheduler sinks the DMA below the stage's MFMAs,
+    // next to the barrier that waits for it, and its latency is exposed.
     if (kb0 < kb1) X3_DMA(lds0, kb0);
     int kb = kb0;
     for (; kb + 2 * KS <= kb1; kb += 2 * KS) {
         __syncthreads();
         X3_DMA(lds1, kb + KS);
+        __builtin_amdgcn_sched_barrier(0);
         X3_STEP(lds0, kb, KS);
         __syncthreads();
         X3_DMA(lds0, kb + 2 * KS < kb1 ? kb + 2 * KS : kb + KS);
+        __builtin_amdgcn_sched_barrier(0);
         X3_STEP(lds1, kb + KS, KS);
     }
     // the launcher makes every k-range a multiple of KS, so at most one
