@@ -44,12 +44,43 @@ __device__ inline float bf2f(unsigned short h)
 
 __device__ inline float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
+// Sum of the 256 threads' values s, valid in thread 0: the tree
+// red[t] += red[t + k], k = 128, 64, ..., 1, with the steps below 64 as
+// shuffles inside the first wave (the same additions in the same order).
+__device__ inline float block_sum_256(float s, float *red)
+{
+    const int t = threadIdx.x;
+    red[t] = s;
+    __syncthreads();
+    if (t < 128) red[t] += red[t + 128];
+    __syncthreads();
+    float v = 0.f;
+    if (t < 64) {
+        v = red[t] + red[t + 64];
+#pragma unroll
+        for (int k = 32; k > 0; k >>= 1) v += __shfl_down(v, k, 64);
+    }
+    return v;
+}
+
 enum { EPI_FWD = 0, EPI_BWD = 1, EPI_WGRAD = 2 };
 
 // C = A B^T on a 64 x 64 tile per workgroup (2 x 2 waves of 32 x 32, 2 x 2
 // blocks of 16 x 16 x 32 each).  M, N multiples of 64, K multiple of 32.
 // The weight-gradient form splits K over blockIdx.z: split z contracts
 // k in [z K, (z + 1) K) and writes its partial sums at G + z gsplit.
+//
+// The operands pass through LDS in K-panels of up to kPanel: every thread
+// loads its share of the panel's 64 rows of A and of B into registers (all
+// loads in flight at once), writes them to LDS, and the waves then run the
+// panel's MFMAs from LDS while the next panel's loads are in flight.  These
+// GEMMs are small (a few MFLOP per workgroup), so one memory latency per
+// panel instead of one per 32 k bounds them.  LDS rows are 32 chunks of
+// 16 bytes (8 k); chunk c of row r sits at c ^ (r & 15), which spreads the
+// 16 rows of an MFMA operand read over all banks.  Each wave accumulates k
+// in ascending order as before.
+constexpr int kPanel = 256;
+constexpr int kPanelChunks = kPanel / 8;
 template <int EPI>
 __global__ void __launch_bounds__(256)
     k_gemm_nt(const unsigned short *__restrict__ A, int64_t lda, const unsigned short *__restrict__ B,
@@ -58,60 +89,133 @@ __global__ void __launch_bounds__(256)
               int64_t ldoT, const float *__restrict__ Zprev, float *__restrict__ G, int64_t ldg,
               int64_t gsplit)
 {
+    __shared__ uint4 sA[64 * kPanelChunks];
+    __shared__ uint4 sB[64 * kPanelChunks];
     if (EPI == EPI_WGRAD) {
         A += (size_t)blockIdx.z * K;
         B += (size_t)blockIdx.z * K;
         G += (size_t)blockIdx.z * gsplit;
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lg = lane >> 4;
-    const int64_t m0 = (int64_t)blockIdx.y * 64 + (wave >> 1) * 32;
-    const int64_t n0 = (int64_t)blockIdx.x * 64 + (wave & 1) * 32;
+    const int64_t mb = (int64_t)blockIdx.y * 64, nb = (int64_t)blockIdx.x * 64;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int64_t m0 = mb + wr, n0 = nb + wc;
     f4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < K; k += 32) {
-        bf16x8 av[2], bv[2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-            av[a] = *reinterpret_cast<const bf16x8 *>(A + (size_t)(m0 + 16 * a + li) * lda + k + 8 * lg);
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-            bv[b] = *reinterpret_cast<const bf16x8 *>(B + (size_t)(n0 + 16 * b + li) * ldb + k + 8 * lg);
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+
+    // panel staging: position q = tid + 256 j of an operand's 64 x cpr chunks
+    // (row q / cpr, chunk q % cpr), j < cpr / 4 <= 8, in named registers (an
+    // array of them is kept in scratch memory by the compiler)
+    uint4 ra0, ra1, ra2, ra3, ra4, ra5, ra6, ra7, rb0, rb1, rb2, rb3, rb4, rb5, rb6, rb7;
+    // a slot beyond the panel re-reads the thread's first chunk (same lines),
+    // so that every slot is assigned
+#define GEMM_LD1(j, k0_, cpr_)                                                               \
+    do {                                                                                     \
+        const int q = tid + 256 * ((j) < (cpr_) / 4 ? (j) : 0), r = q / (cpr_), c = q - r * (cpr_); \
+        ra##j = *reinterpret_cast<const uint4 *>(A + (size_t)(mb + r) * lda + (k0_) + 8 * c); \
+        rb##j = *reinterpret_cast<const uint4 *>(B + (size_t)(nb + r) * ldb + (k0_) + 8 * c); \
+    } while (0)
+#define GEMM_LOAD(k0_)                                                                       \
+    do {                                                                                     \
+        const int cpr_ = min(kPanel, K - (k0_)) / 8;                                         \
+        GEMM_LD1(0, k0_, cpr_); GEMM_LD1(1, k0_, cpr_); GEMM_LD1(2, k0_, cpr_);              \
+        GEMM_LD1(3, k0_, cpr_); GEMM_LD1(4, k0_, cpr_); GEMM_LD1(5, k0_, cpr_);              \
+        GEMM_LD1(6, k0_, cpr_); GEMM_LD1(7, k0_, cpr_);                                      \
+    } while (0)
+#define GEMM_ST1(j)                                                                          \
+    do {                                                                                     \
+        if ((j) < cpr / 4) {                                                                 \
+            const int q = tid + 256 * (j), r = q / cpr, c = q - r * cpr;                     \
+            const int pos = r * kPanelChunks + (c ^ (r & 15));                               \
+            sA[pos] = ra##j;                                                                 \
+            sB[pos] = rb##j;                                                                 \
+        }                                                                                    \
+    } while (0)
+    GEMM_LOAD(0);
+    for (int k0 = 0; k0 < K; k0 += kPanel) {
+        const int cpr = min(kPanel, K - k0) / 8;
+        if (k0 > 0) __syncthreads();  // every wave is done with the previous panel
+        GEMM_ST1(0); GEMM_ST1(1); GEMM_ST1(2); GEMM_ST1(3);
+        GEMM_ST1(4); GEMM_ST1(5); GEMM_ST1(6); GEMM_ST1(7);
+        __syncthreads();
+        if (k0 + kPanel < K) GEMM_LOAD(k0 + kPanel);
+        // k-steps in pairs: both steps' fragment reads are issued before the
+        // first step's MFMAs (each accumulator still takes k in order)
+        const int nks = cpr / 4;
+#define GEMM_FRAGS(ks_, av_, bv_)                                                            \
+    do {                                                                                     \
+        _Pragma("unroll") for (int a = 0; a < 2; ++a) {                                      \
+            const int r = wr + 16 * a + li;                                                  \
+            av_[a] = __builtin_bit_cast(bf16x8, sA[r * kPanelChunks + ((4 * (ks_) + lg) ^ (r & 15))]); \
+        }                                                                                    \
+        _Pragma("unroll") for (int b = 0; b < 2; ++b) {                                      \
+            const int r = wc + 16 * b + li;                                                  \
+            bv_[b] = __builtin_bit_cast(bf16x8, sB[r * kPanelChunks + ((4 * (ks_) + lg) ^ (r & 15))]); \
+        }                                                                                    \
+    } while (0)
+#define GEMM_MFMAS(av_, bv_)                                                                 \
+    do {                                                                                     \
+        _Pragma("unroll") for (int a = 0; a < 2; ++a)                                        \
+            _Pragma("unroll") for (int b = 0; b < 2; ++b)                                    \
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av_[a], bv_[b], acc[a][b], 0, 0, 0); \
+    } while (0)
+        int ks = 0;
+        for (; ks + 1 < nks; ks += 2) {
+            bf16x8 av0[2], bv0[2], av1[2], bv1[2];
+            GEMM_FRAGS(ks, av0, bv0);
+            GEMM_FRAGS(ks + 1, av1, bv1);
+            GEMM_MFMAS(av0, bv0);
+            GEMM_MFMAS(av1, bv1);
+        }
+        if (ks < nks) {
+            bf16x8 av0[2], bv0[2];
+            GEMM_FRAGS(ks, av0, bv0);
+            GEMM_MFMAS(av0, bv0);
+        }
+#undef GEMM_FRAGS
+#undef GEMM_MFMAS
     }
-    // D[i][j]: lane j + 16*(i/4), element i%4
+#undef GEMM_LD1
+#undef GEMM_LOAD
+#undef GEMM_ST1
+    // D[i][j]: lane j + 16*(i/4), element i%4; the lane's 4 elements are 4
+    // consecutive m of one n, so each transposed copy gets one 8-byte store
     const int oj = lane & 15, oi = 4 * (lane >> 4);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < 2; ++b) {
+            const int64_t mr = m0 + 16 * a + oi, n = n0 + 16 * b + oj;
+            unsigned hv[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t m = m0 + 16 * a + oi + e, n = n0 + 16 * b + oj;
+                const int64_t m = mr + e;
                 float v = acc[a][b][e];
                 if (EPI == EPI_FWD) {
                     v += bias[n];
                     Z[(size_t)m * ldo + n] = v;
                     const unsigned short h = f2bf(v * sigm(v));
                     Out[(size_t)m * ldo + n] = h;
-                    if (OutT) OutT[(size_t)n * ldoT + m] = h;
+                    hv[e] = h;
                 } else if (EPI == EPI_BWD) {
                     const float z = Zprev[(size_t)m * ldo + n];
                     const float s = sigm(z);
                     const unsigned short h = f2bf(v * s * (1.0f + z * (1.0f - s)));
                     Out[(size_t)m * ldo + n] = h;
-                    OutT[(size_t)n * ldoT + m] = h;
+                    hv[e] = h;
                 } else {
                     G[(size_t)m * ldg + n] = v;
                 }
             }
+            if (EPI != EPI_WGRAD && OutT)
+                *reinterpret_cast<uint2 *>(OutT + (size_t)n * ldoT + mr) =
+                    make_uint2(hv[0] | (hv[1] << 16), hv[2] | (hv[3] << 16));
+        }
 }
 
 // x[p][k] = bf16(sqrt(2) y[k][p]) for k < M, zero up to dpad; optional x^T.
@@ -129,9 +233,15 @@ __global__ void k_mlp_input(const double *__restrict__ y, int64_t ldy, int M, in
     }
 }
 
-// out[p] = a3[p] . w3 + b3; one wave per point group.  Training mode also
-// forms the loss terms and the head's backward: dout = 2(out - t)/B,
-// delta3 = dout w3 silu'(z3) (both layouts).
+// out[p] = a3[p] . w3 + b3.  A workgroup takes kHeadPts consecutive points
+// (npts is a multiple of kHeadPts), each wave a quarter of them together:
+// lane-strided partial sums over j (per point in ascending j), then a
+// butterfly.  Training mode also forms the loss terms and the head's
+// backward, dout = 2 (out - t) / B and delta3 = dout w3 silu'(z3), written
+// row-major [p][j] and transposed [j][p]: thread j packs its points into
+// 16-byte stores.
+constexpr int kHeadPts = 8;
+constexpr int kHeadWavePts = kHeadPts / 4;
 __global__ void __launch_bounds__(256)
     k_mlp_head(const unsigned short *__restrict__ a3, const float *__restrict__ z3, int h,
                const float *__restrict__ w3, const float *__restrict__ b3, int64_t npts,
@@ -139,84 +249,141 @@ __global__ void __launch_bounds__(256)
                float *__restrict__ sqerr, float *__restrict__ dout,
                unsigned short *__restrict__ d3, unsigned short *__restrict__ d3T, int64_t ldT)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (p >= npts) return;
-    float s = 0.f;
-    for (int j = lane; j < h; j += 64) s += bf2f(a3[(size_t)p * h + j]) * w3[j];
+    __shared__ float gsh[kHeadPts];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t p0 = (int64_t)blockIdx.x * kHeadPts;
+    const int64_t pw = p0 + wave * kHeadWavePts;
+    float sv[kHeadWavePts];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    const float o = s + b3[0];
-    if (lane == 0) out[p] = o;
-    if (!target) return;
-    const float r = o - target[p];
-    const float g = 2.0f * r * inv_batch;
-    if (lane == 0) {
-        sqerr[p] = r * r;
-        dout[p] = g;
-    }
+    for (int i = 0; i < kHeadWavePts; ++i) sv[i] = 0.f;
     for (int j = lane; j < h; j += 64) {
-        const float z = z3[(size_t)p * h + j];
-        const float sg = sigm(z);
-        const unsigned short v = f2bf(g * w3[j] * sg * (1.0f + z * (1.0f - sg)));
-        d3[(size_t)p * h + j] = v;
-        d3T[(size_t)j * ldT + p] = v;
+        const float wj = w3[j];
+#pragma unroll
+        for (int i = 0; i < kHeadWavePts; ++i) sv[i] += bf2f(a3[(size_t)(pw + i) * h + j]) * wj;
+    }
+#pragma unroll
+    for (int i = 0; i < kHeadWavePts; ++i) {
+        float s = sv[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        const float o = s + b3[0];
+        const int64_t p = pw + i;
+        if (lane == 0) {
+            out[p] = o;
+            if (target) {
+                const float r = o - target[p];
+                const float g = 2.0f * r * inv_batch;
+                sqerr[p] = r * r;
+                dout[p] = g;
+                gsh[wave * kHeadWavePts + i] = g;
+            }
+        }
+    }
+    if (!target) return;
+    __syncthreads();
+    for (int j = tid; j < h; j += 256) {
+        const float wj = w3[j];
+        unsigned pk[kHeadPts / 2];
+#pragma unroll
+        for (int pl = 0; pl < kHeadPts; ++pl) {
+            const int64_t p = p0 + pl;
+            const float z = z3[(size_t)p * h + j];
+            const float sg = sigm(z);
+            const unsigned short v = f2bf(gsh[pl] * wj * sg * (1.0f + z * (1.0f - sg)));
+            d3[(size_t)p * h + j] = v;
+            if (pl & 1) pk[pl >> 1] |= (unsigned)v << 16;
+            else pk[pl >> 1] = v;
+        }
+        uint4 *dst = reinterpret_cast<uint4 *>(d3T + (size_t)j * ldT + p0);
+#pragma unroll
+        for (int c = 0; c < kHeadPts / 8; ++c)
+            dst[c] = make_uint4(pk[4 * c], pk[4 * c + 1], pk[4 * c + 2], pk[4 * c + 3]);
     }
 }
 
-// Row sums of a [rows][cols] bf16 matrix (bias gradients from delta^T), and
-// the head's weight gradient sum_p dout[p] a3[p][j].
+// The gradient sums of one step in one launch, each as 256 strided partial
+// sums (unrolled so that the loads are in flight together; the adds keep
+// their order) and a tree: blocks [0, 3h) the hidden layers' bias gradients
+// (row r of layer l's delta^T, block l h + r), blocks [3h, 4h) the head's
+// weight gradient sum_p dout[p] a3[p][j], block 4h its bias gradient
+// sum_p dout[p], block 4h + 1 the batch's summed squared error (loss).
+struct GradSums {
+    const unsigned short *dT[3];  // delta^T of the hidden layers, [h][ld]
+    float *gb[3];                 // their bias gradients
+    const unsigned short *a3T;    // last activations^T, [h][ld]
+    const float *dout, *sqerr;
+    float *gw3, *gb3, *loss;
+};
 __global__ void __launch_bounds__(256)
-    k_rowsum_bf16(const unsigned short *__restrict__ X, int64_t ld, int64_t cols,
-                  float *__restrict__ out)
+    k_grad_sums(GradSums gs, int h, int64_t ld, int64_t batch)
 {
     __shared__ float red[256];
-    const int r = blockIdx.x;
+    const int b = blockIdx.x;
     float s = 0.f;
-    for (int64_t c = threadIdx.x; c < cols; c += 256) s += bf2f(X[(size_t)r * ld + c]);
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
+    if (b < 3 * h) {
+        const int l = b / h, r = b - l * h;
+        const unsigned short *X = gs.dT[l] + (size_t)r * ld;
+#pragma unroll 16
+        for (int64_t c = threadIdx.x; c < batch; c += 256) s += bf2f(X[c]);
+    } else if (b < 4 * h) {
+        const unsigned short *X = gs.a3T + (size_t)(b - 3 * h) * ld;
+#pragma unroll 16
+        for (int64_t p = threadIdx.x; p < batch; p += 256) s += gs.dout[p] * bf2f(X[p]);
+    } else if (b == 4 * h) {
+#pragma unroll 16
+        for (int64_t p = threadIdx.x; p < batch; p += 256) s += gs.dout[p] * 1.0f;
+    } else {
+#pragma unroll 16
+        for (int64_t p = threadIdx.x; p < batch; p += 256) s += gs.sqerr[p];
     }
-    if (threadIdx.x == 0) out[r] = red[0];
-}
-
-__global__ void __launch_bounds__(256)
-    k_head_grad(const unsigned short *__restrict__ a3T, int64_t ldT, const float *__restrict__ dout,
-                int64_t batch, float *__restrict__ gw3, float *__restrict__ gb3)
-{
-    __shared__ float red[256];
-    const int j = blockIdx.x;  // j == h gives the bias
-    float s = 0.f;
-    for (int64_t p = threadIdx.x; p < batch; p += 256)
-        s += dout[p] * (j < (int)gridDim.x - 1 ? bf2f(a3T[(size_t)j * ldT + p]) : 1.0f);
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
-    }
+    s = block_sum_256(s, red);
     if (threadIdx.x == 0) {
-        if (j < (int)gridDim.x - 1) gw3[j] = red[0];
-        else gb3[0] = red[0];
+        if (b < 3 * h) gs.gb[b / h][b % h] = s;
+        else if (b < 4 * h) gs.gw3[b - 3 * h] = s;
+        else if (b == 4 * h) gs.gb3[0] = s;
+        else gs.loss[0] = s;
     }
 }
 
-// Adam (Kingma & Ba 2015) on the FP32 master parameters.
-__global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ m,
-                       float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
-                       float c1, float c2)
+// One training update in one launch: the split-K partial gradients summed in
+// split order (as k_grad_reduce), the Adam step (Kingma & Ba 2015) on the
+// FP32 master parameters, and the BF16 copies W[o][i], W^T[i][o] of an
+// updated hidden-layer weight (as k_to_bf16).  One parameter per thread: a
+// 2-D tiling that writes W^T in longer runs leaves too few workgroups (77
+// for the hexamer network) and was 5x slower.
+struct WeightCopies {
+    int64_t w_off[3];
+    int in[3];
+    int out;
+    unsigned short *Wb[3], *WbT[3];
+};
+__global__ void __launch_bounds__(256)
+    k_adam_update(float *__restrict__ P, const float *__restrict__ part, int nsplit, float *__restrict__ G,
+                  float *__restrict__ m, float *__restrict__ v, int64_t n, float lr, float b1, float b2,
+                  float eps, float c1, float c2, WeightCopies wc)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float g = G[i];
+    float g = part[i];
+#pragma unroll 16
+    for (int sp = 1; sp < nsplit; ++sp) g += part[(size_t)sp * n + i];
+    G[i] = g;
     const float mi = b1 * m[i] + (1.f - b1) * g;
     const float vi = b2 * v[i] + (1.f - b2) * g * g;
     m[i] = mi;
     v[i] = vi;
-    P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    const float p = P[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    P[i] = p;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        const int64_t e = i - wc.w_off[l];
+        if (e >= 0 && e < (int64_t)wc.out * wc.in[l]) {
+            const int r = (int)(e / wc.in[l]), c = (int)(e - (int64_t)r * wc.in[l]);
+            const unsigned short h = f2bf(p);
+            wc.Wb[l][e] = h;
+            wc.WbT[l][(size_t)c * wc.out + r] = h;
+        }
+    }
 }
 
 // G[i] = sum over splits s of part[s][i], in split order.
@@ -287,20 +454,6 @@ __global__ void k_mlp_targets(const double *__restrict__ E, const double *__rest
         hsum += yk * (omega[k] * yk);
     }
     raw[i] = E[i] / kAutokcalpmol - 0.5 * hsum - V0;
-}
-
-__global__ void k_sum_f32(const float *__restrict__ x, int64_t n, float *__restrict__ out)
-{
-    __shared__ float red[256];
-    float s = 0.f;
-    for (int64_t i = threadIdx.x; i < n; i += 256) s += x[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[0] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -477,7 +630,7 @@ static hipError_t eval_f(scp3g_ctx *ctx, MlpState *s, const double *y, int64_t l
         hipMemsetAsync(s->x + (size_t)np * s->dpad, 0, (size_t)(npad - np) * s->dpad * 2, ctx->stream);
     if ((e = inputs(ctx, s, y, ldy, np, s->x, nullptr, 0)) != hipSuccess) return e;
     if ((e = forward(ctx, s, s->x, npad, false, nullptr)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mlp_head, dim3((unsigned)((npad + 3) / 4)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_mlp_head, dim3((unsigned)(npad / kHeadPts)), dim3(256), 0, ctx->stream,
                        s->act[2], s->z[2], s->h, s->P + s->w3_off, s->P + s->b3_off, npad,
                        (const float *)nullptr, 0.f, s->out, (float *)nullptr, (float *)nullptr,
                        (unsigned short *)nullptr, (unsigned short *)nullptr, (int64_t)0);
@@ -499,26 +652,20 @@ hipError_t launch_mlp_weights(scp3g_ctx *ctx, const double *y, int64_t ldy, int6
 }
 
 // one forward + backward on batch [b0, b0 + B) of the training set; leaves
-// the gradient in s->G and the batch's summed squared error in *lossd.
-static hipError_t grad_step(scp3g_ctx *ctx, MlpState *s, int64_t b0, int64_t B, float *lossd)
+// the split-K partial gradients in s->Gpart (and, with reduce, their sum in
+// s->G) and the batch's summed squared error in *lossd.
+static hipError_t grad_step(scp3g_ctx *ctx, MlpState *s, int64_t b0, int64_t B, float *lossd, bool reduce)
 {
     hipError_t e;
     const unsigned short *x = s->Xtr + (size_t)b0 * s->dpad;
     const unsigned short *xT = s->XtrT + b0;  // ld = ntrain
     if ((e = forward(ctx, s, x, B, true, nullptr)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mlp_head, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_mlp_head, dim3((unsigned)(B / kHeadPts)), dim3(256), 0, ctx->stream,
                        s->act[2], s->z[2], s->h, s->P + s->w3_off, s->P + s->b3_off, B,
                        s->Ttr + b0, 1.0f / (float)B, s->out, s->sq, s->dout, s->dl[2], s->dlT[2],
                        (int64_t)s->cap);
-    hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, ctx->stream, s->sq, B, lossd);
-    // head: w3, b3
-    hipLaunchKernelGGL(k_head_grad, dim3((unsigned)(s->h + 1)), dim3(256), 0, ctx->stream,
-                       s->actT[2], (int64_t)s->cap, s->dout, B, s->Gpart + s->w3_off, s->Gpart + s->b3_off);
     for (int l = 2; l >= 0; --l) {
         const Layer &Ly = s->L[l];
-        // bias gradient: row sums of delta^T
-        hipLaunchKernelGGL(k_rowsum_bf16, dim3((unsigned)Ly.out), dim3(256), 0, ctx->stream,
-                           s->dlT[l], (int64_t)s->cap, B, s->Gpart + Ly.b_off);
         // weight gradient G[o][i] = sum_p delta[p][o] a_in[p][i], the batch
         // split so that the grid has >= 256 workgroups
         const unsigned short *inT = l > 0 ? s->actT[l - 1] : xT;
@@ -549,8 +696,24 @@ static hipError_t grad_step(scp3g_ctx *ctx, MlpState *s, int64_t b0, int64_t B, 
                                s->z[l - 1], (float *)nullptr, (int64_t)0, (int64_t)0);
         }
     }
-    hipLaunchKernelGGL(k_grad_reduce, dim3((unsigned)((s->nparam + 255) / 256)), dim3(256), 0, ctx->stream,
-                       s->Gpart, s->gsplit, s->nparam, s->G);
+    // bias gradients (row sums of each layer's delta^T), the head's gradient
+    // and the loss
+    GradSums gs;
+    for (int l = 0; l < 3; ++l) {
+        gs.dT[l] = s->dlT[l];
+        gs.gb[l] = s->Gpart + s->L[l].b_off;
+    }
+    gs.a3T = s->actT[2];
+    gs.dout = s->dout;
+    gs.sqerr = s->sq;
+    gs.gw3 = s->Gpart + s->w3_off;
+    gs.gb3 = s->Gpart + s->b3_off;
+    gs.loss = lossd;
+    hipLaunchKernelGGL(k_grad_sums, dim3((unsigned)(4 * s->h + 2)), dim3(256), 0, ctx->stream, gs, s->h,
+                       (int64_t)s->cap, B);
+    if (reduce)
+        hipLaunchKernelGGL(k_grad_reduce, dim3((unsigned)((s->nparam + 255) / 256)), dim3(256), 0,
+                           ctx->stream, s->Gpart, s->gsplit, s->nparam, s->G);
     return hipGetLastError();
 }
 
@@ -788,18 +951,27 @@ int scp3g_mlp_train(scp3g_ctx *ctx, int steps, int batch, double lr, float *loss
     if ((rc = check_hip(ctx, ensure_batch(s, batch), "alloc batch"))) return rc;
     const int64_t nb = s->ntrain / batch;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    WeightCopies wc;
+    wc.out = s->h;
+    for (int l = 0; l < 3; ++l) {
+        wc.w_off[l] = s->L[l].w_off;
+        wc.in[l] = s->L[l].in;
+        wc.Wb[l] = s->L[l].Wb;
+        wc.WbT[l] = s->L[l].WbT;
+    }
     hipEvent_t ev;
     timer_begin(ctx, "mlp_train", &ev);
     std::vector<float> losses;
     for (int it = 0; it < steps; ++it) {
         const int64_t b0 = (it % nb) * batch;
         float *ld = s->lossd + (it % (64 * 1024));
-        hipError_t e = grad_step(ctx, s, b0, batch, ld);
+        hipError_t e = grad_step(ctx, s, b0, batch, ld, false);
         ++s->step;
         const float c1 = 1.f - std::pow(b1, (float)s->step), c2 = 1.f - std::pow(b2, (float)s->step);
-        hipLaunchKernelGGL(k_adam, dim3((unsigned)((s->nparam + 255) / 256)), dim3(256), 0, ctx->stream,
-                           s->P, s->G, s->m, s->v, s->nparam, (float)lr, b1, b2, eps, c1, c2);
-        if (e == hipSuccess) e = refresh_bf16(ctx, s);
+        hipLaunchKernelGGL(k_adam_update, dim3((unsigned)((s->nparam + 255) / 256)), dim3(256), 0,
+                           ctx->stream, s->P, s->Gpart, s->gsplit, s->G, s->m, s->v, s->nparam, (float)lr, b1,
+                           b2, eps, c1, c2, wc);
+        if (e == hipSuccess) e = hipGetLastError();
         if ((rc = check_hip(ctx, e, "train step"))) return rc;
         if (loss_host && ((it + 1) % (64 * 1024) == 0 || it + 1 == steps)) {
             const int64_t first = (it / (64 * 1024)) * (64 * 1024);
@@ -823,7 +995,7 @@ int scp3g_mlp_grad(scp3g_ctx *ctx, int batch, float *grad_host, float *loss)
     MlpState *s = ctx->mlp;
     if (!s->Xtr || batch <= 0 || batch % 64 || batch > s->ntrain) return fail(ctx, -4, "bad batch");
     if ((rc = check_hip(ctx, ensure_batch(s, batch), "alloc batch"))) return rc;
-    if ((rc = check_hip(ctx, grad_step(ctx, s, 0, batch, s->lossd), "grad"))) return rc;
+    if ((rc = check_hip(ctx, grad_step(ctx, s, 0, batch, s->lossd, true), "grad"))) return rc;
     hipStreamSynchronize(ctx->stream);
     if (grad_host) hipMemcpy(grad_host, s->G, (size_t)s->nparam * 4, hipMemcpyDeviceToHost);
     if (loss) {

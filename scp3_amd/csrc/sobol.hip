@@ -119,7 +119,7 @@ __device__ __forceinline__ double bsm(double u)
     return x;
 }
 
-constexpr int kSobolPPT = 16;  // points per lane (strided by 64)
+constexpr int kSobolPPT = 4;  // points per lane (strided by 64)
 
 // grid: x = groups of (waves per block * 64 * PPT) points, y = dimension
 __global__ void __launch_bounds__(256)

@@ -113,6 +113,14 @@ def test_adam_steps_match_restatement(gpu_ctx_factory, hexamer):
             assert big.sum() > 1000
             np.testing.assert_allclose(np.abs(P_dev - P0)[big], lr, rtol=2e-2)
         P = P_dev.copy()
+    # the update also rewrites the BF16 copies W and W^T that the forward and
+    # backward GEMMs read: the gradient through them equals the gradient after
+    # the copies are re-derived from the FP32 parameters by mlp_set_params
+    g_upd, l_upd = ctx.mlp_grad(B)
+    ctx.mlp_set_params(P, np.array([0.0, 0.0, 1.0]))
+    g_set, l_set = ctx.mlp_grad(B)
+    np.testing.assert_array_equal(g_upd, g_set)
+    assert l_upd == l_set
 
 
 @pytest.mark.gpu
