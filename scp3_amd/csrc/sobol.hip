@@ -119,9 +119,11 @@ __device__ __forceinline__ double bsm(double u)
     return x;
 }
 
-constexpr int kSobolPPT = 4;  // points per lane (strided by 64)
-
-// grid: x = groups of (waves per block * 64 * PPT) points, y = dimension
+// grid: x = groups of (waves per block * 64 * PPT) points, y = dimension;
+// PPT points per lane (strided by 64): each lane pays ~log2(seed) XORs for
+// its starting state, so large tiles take 16 points per lane and small ones
+// 4, to keep enough waves in flight (launch_sobol)
+template <int PPT>
 __global__ void __launch_bounds__(256)
     k_sobol(const uint64_t *__restrict__ dirs, const uint64_t *__restrict__ shift, int64_t skip,
             int64_t npts, int halve, double *__restrict__ y, int64_t ldy, uint64_t *__restrict__ ints)
@@ -132,7 +134,7 @@ __global__ void __launch_bounds__(256)
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int64_t i0 =
-        ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * (64 * kSobolPPT) + lane;
+        ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * (64 * PPT) + lane;
     if (i0 >= npts) return;
     uint64_t s = (uint64_t)(skip + i0);
     uint64_t g = s ^ (s >> 1);
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(256)
     const uint64_t v5 = v[5];
     double *__restrict__ yd = y + (size_t)d * ldy;
     uint64_t *__restrict__ id = ints ? ints + (size_t)d * ldy : nullptr;
-    for (int k = 0; k < kSobolPPT; ++k) {
+    for (int k = 0; k < PPT; ++k) {
         const int64_t i = i0 + 64 * k;
         if (i >= npts) break;
         const double u = (double)st * 0x1p-62;  // recipd, sobol.cpp:27780,27846
@@ -163,13 +165,19 @@ hipError_t launch_sobol(scp3g_ctx *ctx, const uint64_t *dirs, int dims, int64_t 
 {
     if (npts <= 0) return hipSuccess;
     const int waves = 4;
-    const int64_t per_block = (int64_t)waves * 64 * kSobolPPT;
-    dim3 grid((unsigned)((npts + per_block - 1) / per_block), (unsigned)dims);
     // the context's own table carries the digital shifts of a scrambled
     // sequence behind its direction integers (scp3g_set_scramble)
     const uint64_t *shift = (dirs == ctx->d_sobol && ctx->scramble_on) ? dirs + (size_t)dims * kSobolCols : nullptr;
-    hipLaunchKernelGGL(k_sobol, grid, dim3(64 * waves), 0, ctx->stream, dirs, shift, skip, npts,
-                       halve, y, ldy, ints);
+    // 16 points per lane when that still gives >= 8 waves per SIMD (8192)
+    const int64_t groups16 = (npts + waves * 64 * 16 - 1) / (waves * 64 * 16);
+    if (groups16 * waves * dims >= 8192) {
+        hipLaunchKernelGGL(k_sobol<16>, dim3((unsigned)groups16, (unsigned)dims), dim3(64 * waves), 0, ctx->stream,
+                           dirs, shift, skip, npts, halve, y, ldy, ints);
+    } else {
+        const int64_t groups4 = (npts + waves * 64 * 4 - 1) / (waves * 64 * 4);
+        hipLaunchKernelGGL(k_sobol<4>, dim3((unsigned)groups4, (unsigned)dims), dim3(64 * waves), 0, ctx->stream,
+                           dirs, shift, skip, npts, halve, y, ldy, ints);
+    }
     return hipGetLastError();
 }
 
