@@ -99,9 +99,8 @@ int scp3g_set_pes(scp3g_ctx *ctx, const char *name);
 int scp3g_set_precision(scp3g_ctx *ctx, int mode);
 /* SYRK kernels: 1 (default) = signed 256 x 256-tile split-BF16 kernel on
  * X = sqrt|w| B, 8 waves on 16x16x32 MFMA, operands moved to LDS by DMA;
- * 2 = the same tile with 4 waves on 32x32x16 MFMA, register-staged;
- * 0 = 64 x 64-tile direct-load kernels on B and w.  Any nonzero value
- * selects the LDS-tiled FP64 kernel in the FP64 form. */
+ * 0 = 64 x 64-tile direct-load kernels on B and w.  1 also selects the
+ * LDS-tiled FP64 kernel in the FP64 form. */
 int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast);
 /*
  * SCP3G_EST_ENERGY (default), SCP3G_EST_GRADIENT or SCP3G_EST_GRADIENT_STEIN

@@ -488,9 +488,8 @@ int scp3g_set_precision(scp3g_ctx *ctx, int mode)
 int scp3g_set_mixed_kernel(scp3g_ctx *ctx, int fast)
 {
     if (!ctx) return -1;
-    if (fast < 0 || fast > 2) return fail(ctx, -4, "mixed kernel must be 0, 1 or 2");
+    if (fast < 0 || fast > 1) return fail(ctx, -4, "mixed kernel must be 0 or 1");
     ctx->mixed_fast = fast != 0;
-    if (fast) ctx->x3_variant = fast;
     return 0;
 }
 

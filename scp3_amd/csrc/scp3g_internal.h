@@ -62,10 +62,6 @@ struct scp3g_ctx {
     // syrk64_fast.hip: 128 x 128 FP64) instead of the 64 x 64 direct-load
     // kernels (syrk.hip, syrk_mixed.hip)
     bool mixed_fast = true;
-    // signed split-BF16 SYRK variant: 1 = 8 waves of 128 x 64 on 16x16x32
-    // MFMA, operands to LDS by DMA; 2 = 4 waves of 128 x 128 on 32x32x16
-    // MFMA, register-staged
-    int x3_variant = 1;
     // control variate, mixed form: the surrogate term with hi.hi products only
     bool cv_one_term = false;
     // SCP3G_EST_ENERGY (the -G estimator) or one of the gradient forms

@@ -53,7 +53,6 @@ __device__ __forceinline__ bf16x8 lds_frag_xor(const uint4 &src, u32x4 f)
     const u32x4 v = *reinterpret_cast<const u32x4 *>(&src);
     return __builtin_bit_cast(bf16x8, v ^ f);
 }
-constexpr int kMaxSignWords = 4096;  // 32-point sign words per split (131072 points)
 
 __device__ __forceinline__ unsigned short bf16_rne_u(float f)
 {
@@ -417,244 +416,36 @@ __device__ __forceinline__ void kahan_add2(double &h, double &c, double v)
     h = t;
 }
 
-typedef float f16v __attribute__((ext_vector_type(16)));
-
-// Variant on v_mfma_f32_32x32x16_bf16 with 4 waves of 128 x 128.
+// Work item of k_syrk_x3: x = row block | column block << 16; y = the
+// k-range's index (k-blocks y kbs .. min((y + 1) kbs, nkb)) when w >= 0; w =
+// the slot of the FP32 partial that the item writes, or -1: the item covers
+// the whole K range and Kahan-adds its tile into H.
 //
-// * Workgroup: 4 waves (2 x 2) on a 256 x 256 tile; wave (wr, wc) owns H
-//   rows wr*128.. and columns wc*128.. as 4 x 4 blocks of 32 x 32 (256 FP32
-//   accumulators per lane, in AGPRs).  Per 32-point step a CU reads 128 KiB
-//   of operands from LDS against 192 KiB for the 8-wave 128 x 64 layout.
-//   The MFMA is fed the column side as its A operand and the row side as
-//   its B operand, so its result D(j, i) = H(i, j) puts 32 consecutive H
-//   rows in the 32 lanes of a half-wave: column-major H and the split-K
-//   partials are written in 128/256-byte runs.
-// * Operand layouts (checked on gfx950): lane l supplies row/column l % 32
-//   and contraction indices 8 (l / 32) .. +7; result element e of lane l is
-//   D(8 (e / 4) + 4 (l / 32) + e % 4, l % 32).
-// * LDS: [stage][plane Ah, Al, Bh, Bl][256 rows x 4 chunks of 16 B], double
-//   buffered; chunk c of row r at position c ^ ((r >> 2) & 3), free of bank
-//   conflicts for these reads (SQ_LDS_BANK_CONFLICT).  Step kb+1 is loaded
-//   into named registers (16 chunks per thread; an array would be promoted
-//   to LDS scratch) at the start of step kb and stored to the idle stage
-//   after its MFMAs; sched_barriers keep that order.
-// * Per 16-point substep u a lane reads chunk 2u + l/32 of its rows; the
-//   weight signs are applied to the column-side operand by a sign-bit XOR.
-template <int TERMS>
-__global__ void __launch_bounds__(256, 1)
-    k_syrk_x3m(const uint4 *__restrict__ Xh, const uint4 *__restrict__ Xl,
-               const unsigned *__restrict__ sgn, int64_t rows_pad, const int4 *__restrict__ work,
-               int ntiles, int kb_per_split, int nkb, int64_t n, double *__restrict__ H,
-               double *__restrict__ Hc, float *__restrict__ part)
-{
-    __shared__ uint4 lds[2][4][kFT * 4];
-    // the sign words of the workgroup's K range, read once into LDS
-    __shared__ unsigned sgl[kMaxSignWords];
-    const int4 wk = work[blockIdx.x];
-    const int split = wk.z, tile = wk.w;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int l32 = lane & 31, kh = lane >> 5;
-    const int kb0 = split * kb_per_split;
-    int kb1 = kb0 + kb_per_split;
-    if (kb1 > nkb) kb1 = nkb;
-    const int64_t rowA = (int64_t)wk.x * kFT, rowB = (int64_t)wk.y * kFT;
-    const size_t planeStride = (size_t)rows_pad * 4;  // uint4 per 32-point chunk plane
-
-    // staging: thread tid moves chunks q_j = tid + 256 j (row q / 4, chunk
-    // q % 4) of every needed plane; global offsets g_j, LDS positions p_j
-    const int g0 = tid, g1 = tid + 256, g2 = tid + 512, g3 = tid + 768;
-#define X3M_POS(q) (((q) >> 2) * 4 + (((q) & 3) ^ (((q) >> 4) & 3)))
-    const int p0 = X3M_POS(g0), p1 = X3M_POS(g1), p2 = X3M_POS(g2), p3 = X3M_POS(g3);
-#undef X3M_POS
-    const uint4 *pAh = Xh + (size_t)rowA * 4, *pAl = Xl + (size_t)rowA * 4;
-    const uint4 *pBh = Xh + (size_t)rowB * 4, *pBl = Xl + (size_t)rowB * 4;
-    uint4 ta0, ta1, ta2, ta3, tb0, tb1, tb2, tb3, tc0, tc1, tc2, tc3, td0, td1, td2, td3;
-#define X3M_LD4(P, ptr, base)           \
-    t##P##0 = (ptr)[(base) + g0];       \
-    t##P##1 = (ptr)[(base) + g1];       \
-    t##P##2 = (ptr)[(base) + g2];       \
-    t##P##3 = (ptr)[(base) + g3]
-#define X3M_ST4(P, sg, slot)            \
-    lds[sg][slot][p0] = t##P##0;        \
-    lds[sg][slot][p1] = t##P##1;        \
-    lds[sg][slot][p2] = t##P##2;        \
-    lds[sg][slot][p3] = t##P##3
-#define X3M_LD(kbv)                                            \
-    do {                                                       \
-        const size_t base_ = (size_t)(kbv) * planeStride;      \
-        X3M_LD4(a, pAh, base_);                                \
-        X3M_LD4(c, pBh, base_);                                \
-        if (TERMS == 3) {                                      \
-            X3M_LD4(b, pAl, base_);                            \
-            X3M_LD4(d, pBl, base_);                            \
-        }                                                      \
-    } while (0)
-#define X3M_ST(sg)                                             \
-    do {                                                       \
-        X3M_ST4(a, sg, 0);                                     \
-        X3M_ST4(c, sg, 2);                                     \
-        if (TERMS == 3) {                                      \
-            X3M_ST4(b, sg, 1);                                 \
-            X3M_ST4(d, sg, 3);                                 \
-        }                                                      \
-    } while (0)
-
-    f16v acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
-
-    if (kb0 < kb1) {
-        X3M_LD(kb0);
-        X3M_ST(0);
-        for (int i = tid; i < kb1 - kb0; i += 256) sgl[i] = sgn[kb0 + i];
-    }
-    __syncthreads();
-    int s = 0;
-    for (int kb = kb0; kb < kb1; ++kb) {
-        // next step's chunks (the last step re-reads its own, unused)
-        X3M_LD(kb + 1 < kb1 ? kb + 1 : kb);
-        __builtin_amdgcn_sched_barrier(0);
-        const unsigned sw = sgl[kb - kb0];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int c = 2 * u + kh;
-            // column side (MFMA A operand) takes the weight signs
-            const unsigned bits = (sw >> (16 * u + 8 * kh)) & 0xffu;
-            uint4 fl;
-            fl.x = ((bits & 1u) << 15) | (((bits >> 1) & 1u) << 31);
-            fl.y = (((bits >> 2) & 1u) << 15) | (((bits >> 3) & 1u) << 31);
-            fl.z = (((bits >> 4) & 1u) << 15) | (((bits >> 5) & 1u) << 31);
-            fl.w = (((bits >> 6) & 1u) << 15) | (((bits >> 7) & 1u) << 31);
-            bf16x8 ch[4], cl[4], rh[4], rl[4];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int r = wc * 128 + 32 * b + l32;
-                const int p = r * 4 + (c ^ ((r >> 2) & 3));
-                uint4 v = lds[s][2][p];
-                v.x ^= fl.x; v.y ^= fl.y; v.z ^= fl.z; v.w ^= fl.w;
-                ch[b] = __builtin_bit_cast(bf16x8, v);
-                if (TERMS == 3) {
-                    uint4 vl = lds[s][3][p];
-                    vl.x ^= fl.x; vl.y ^= fl.y; vl.z ^= fl.z; vl.w ^= fl.w;
-                    cl[b] = __builtin_bit_cast(bf16x8, vl);
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int r = wr * 128 + 32 * a + l32;
-                const int p = r * 4 + (c ^ ((r >> 2) & 3));
-                rh[a] = __builtin_bit_cast(bf16x8, lds[s][0][p]);
-                if (TERMS == 3) rl[a] = __builtin_bit_cast(bf16x8, lds[s][1][p]);
-            }
-#pragma unroll
-            for (int term = 0; term < TERMS; ++term)
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        const bf16x8 x = term == 2 ? cl[b] : ch[b];
-                        const bf16x8 y = term == 1 ? rl[a] : rh[a];
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc[a][b], 0, 0, 0);
-                    }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        X3M_ST(s ^ 1);
-        __syncthreads();
-        s ^= 1;
-    }
-#undef X3M_LD4
-#undef X3M_ST4
-#undef X3M_LD
-#undef X3M_ST
-
-    // D(j, i) of block (a, b): element e of lane l has j = 8 (e/4) + 4 kh + e%4
-    // (column within the block), i = l32 (row within the block).  One block
-    // at a time (sched_barrier): read out together, the 256 accumulators
-    // would not fit the VGPRs next to the addresses.
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int tr = wr * 128 + 32 * a + l32;
-                const int tc = wc * 128 + 32 * b + 8 * (e >> 2) + 4 * kh + (e & 3);
-                const int64_t row = rowA + tr, col = rowB + tc;
-                const float v = acc[a][b][e];
-                if (part) {
-                    part[((size_t)split * ntiles + tile) * (kFT * kFT) + (size_t)tc * kFT + tr] = v;
-                } else if (row <= col && col < n) {
-                    const size_t idx = (size_t)col * n + row;
-                    double h = H[idx], cc = Hc[idx];
-                    kahan_add2(h, cc, (double)v);
-                    H[idx] = h;
-                    Hc[idx] = cc;
-                }
-            }
-        }
-}
-
-// split-K partials of k_syrk_x3m: [split][tile][column][row] (column-major
-// within the tile); same fixed summation order as k_syrk_x3_fold
-__global__ void __launch_bounds__(256)
-    k_syrk_x3m_fold(const float *__restrict__ part, const int2 *__restrict__ tiles, int ntiles,
-                    int nsplit, int64_t n, double *__restrict__ H, double *__restrict__ Hc)
-{
-    const int tile = blockIdx.x;
-    const int2 t = tiles[tile];
-    const int e = blockIdx.y * 256 + threadIdx.x;
-    const int tc = e / kFT, tr = e % kFT;
-    const int64_t row = (int64_t)t.x * kFT + tr, col = (int64_t)t.y * kFT + tc;
-    if (row > col || col >= n) return;
-    const size_t stride = (size_t)ntiles * (kFT * kFT);
-    const float *p = part + (size_t)tile * (kFT * kFT) + e;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int sp = 0;
-    for (; sp + 4 <= nsplit; sp += 4) {
-        s0 += (double)p[(size_t)sp * stride];
-        s1 += (double)p[(size_t)(sp + 1) * stride];
-        s2 += (double)p[(size_t)(sp + 2) * stride];
-        s3 += (double)p[(size_t)(sp + 3) * stride];
-    }
-    for (; sp < nsplit; ++sp) s0 += (double)p[(size_t)sp * stride];
-    const double sum = (s0 + s1) + (s2 + s3);
-    const size_t idx = (size_t)col * n + row;
-    double h = H[idx], c = Hc[idx];
-    kahan_add2(h, c, sum);
-    H[idx] = h;
-    Hc[idx] = c;
-}
-
 // TERMS = 3: hi.hi + hi.lo + lo.hi; TERMS = 1: hi.hi only (relative error
 // ~2^-8 per product; used for the surrogate term of the control variate)
 template <int TERMS>
 __global__ void __launch_bounds__(512, 1)
     k_syrk_x3(const uint4 *__restrict__ Xh, const uint4 *__restrict__ Xl,
               const unsigned *__restrict__ sgn, int64_t rows_pad, const int4 *__restrict__ work,
-              int ntiles, int kb_per_split, int nkb, int64_t n, double *__restrict__ H,
-              double *__restrict__ Hc, float *__restrict__ part)
+              int kbs, int nkb, int64_t n, double *__restrict__ H, double *__restrict__ Hc,
+              float *__restrict__ part)
 {
     // two stages [plane Ah, Al, Bh, Bl][row * 4 + swizzled chunk], as two
     // arrays: the steps are unrolled by two so that every LDS access names
     // its stage and the compiler does not wait for the other stage's DMA
     __shared__ uint4 lds0[4][kFT * 4];
     __shared__ uint4 lds1[4][kFT * 4];
-    // work item: tile (row block, column block), split, canonical tile index
     const int4 wk = work[blockIdx.x];
-    const int2 t = make_int2(wk.x, wk.y);
-    const int split = wk.z, tile = wk.w;
+    const int2 t = make_int2(wk.x & 0xffff, wk.x >> 16);
+    const int out = wk.w;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 2, wc = wave & 3;
     const int li = lane & 15, lg = lane >> 4;
-    const int kb0 = split * kb_per_split;
-    int kb1 = kb0 + kb_per_split;
-    if (kb1 > nkb) kb1 = nkb;
+    int kb0 = 0, kb1 = nkb;
+    if (out >= 0) {
+        kb0 = wk.y * kbs;
+        kb1 = kb0 + kbs < nkb ? kb0 + kbs : nkb;
+    }
     const size_t rowA = (size_t)t.x * kFT, rowB = (size_t)t.y * kFT;
     const size_t planeStride = (size_t)rows_pad * 4;  // uint4 per 32-point chunk plane
 
@@ -664,34 +455,36 @@ __global__ void __launch_bounds__(512, 1)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 
-    // Operands move global -> LDS by DMA (global_load_lds_dwordx4), one stage
-    // ahead, in 1 KiB blocks of 16 rows of one plane slot, 64 blocks (8 per
-    // wave) per stage.  TERMS = 3: a stage is one 32-point k-block, slots
-    // A-hi, A-lo, B-hi, B-lo.  TERMS = 1 needs only the hi planes, so a stage
-    // holds two k-blocks, slots A-hi[kb], A-hi[kb + 1], B-hi[kb], B-hi[kb + 1]
-    // (half the barriers per MFMA; 9% faster at 20825 states).  Lane l of a
-    // block writes LDS position 16 l, i.e. row l >> 2, slot l & 3, which holds
-    // chunk (l & 3) ^ ((row >> 1) & 3) of that row; so each lane fetches that
-    // chunk, and a block is still one contiguous KiB of global memory.
+    // Operands move global -> LDS by DMA, one stage ahead, in 1 KiB blocks
+    // of 16 rows of one plane slot, 64 blocks (8 per wave) per stage.
+    // TERMS = 3: a stage is one 32-point k-block, slots A-hi, A-lo, B-hi,
+    // B-lo.  TERMS = 1 needs only the hi planes, so a stage holds two
+    // k-blocks, slots A-hi[kb], A-hi[kb + 1], B-hi[kb], B-hi[kb + 1] (half the
+    // barriers per MFMA).  Lane l of a block writes LDS position 16 l, i.e.
+    // row l >> 2, slot l & 3, which holds chunk (l & 3) ^ ((row >> 1) & 3) of
+    // that row; so each lane fetches that chunk, and a block is still one
+    // contiguous KiB of global memory.
+    //
+    // The DMA is buffer_load ... lds (MUBUF), not global_load_lds: the
+    // compiler models global_load_lds as a FLAT access that may touch LDS,
+    // and while one is in flight it turns every LDS wait into a full drain
+    // (lgkmcnt(0)), which would serialise the fragment reads below.  A
+    // wave's 8 blocks lie in one plane slot (blocks 8 wv .. 8 wv + 7 of the
+    // 64, 16 per slot): the buffer resource's base is that plane at the
+    // stage's k-block, the lane's byte offset is one VGPR, block i adds
+    // 1 KiB through soffset, and the bound is one k-block of the plane.
     constexpr int KS = TERMS == 3 ? 1 : 2;  // k-blocks per stage
-    // Row r = rb 16 + (lane >> 2) of a block has (r >> 1) & 3 = (lane >> 3) & 3,
-    // so the lane's chunk offset within a block is the same for all 8 blocks
-    // and the rest of each source address and LDS position is wave-uniform
-    // (scalar registers: 8 per-lane 64-bit pointers would crowd the VGPRs
-    // that hold the accumulators).
     const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int wslot = wv >> 1;
+    // row r = rb 16 + (lane >> 2) of a block has (r >> 1) & 3 = (lane >> 3) & 3
     const int lane_off = (lane >> 2) * 4 + ((lane & 3) ^ ((lane >> 3) & 3));
-    const uint4 *gsrc[8];
+    const unsigned short *wplane =
+        reinterpret_cast<const unsigned short *>(TERMS == 3 ? ((wslot & 1) ? Xl : Xh) : Xh);
+    const int voff = (int)(((wslot < 2 ? rowA : rowB) + (size_t)(wv & 1) * 128) * 64) + lane_off * 16;
+    const int nrec = (int)(rows_pad * 64);
     int loff[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int blk = wv * 8 + i;
-        const int slot = blk >> 4;
-        const int rb = blk & 15;
-        const size_t row0 = (slot < 2 ? rowA : rowB) + (size_t)rb * 16;
-        gsrc[i] = (TERMS == 3 ? ((slot & 1) ? Xl : Xh) : Xh) + row0 * 4;
-        loff[i] = slot * (kFT * 4) + rb * 64;
-    }
+    for (int i = 0; i < 8; ++i) loff[i] = wslot * (kFT * 4) + ((wv & 1) * 8 + i) * 64;
     // DMA of the stage that starts at k-block kbv (the second k-block of a
     // TERMS = 1 stage is clamped to the last one of the range; it is then
     // loaded but not used)
@@ -699,49 +492,47 @@ __global__ void __launch_bounds__(512, 1)
     do {                                                                                          \
         const int k0_ = (kbv);                                                                    \
         const int k1_ = k0_ + 1 < kb1 ? k0_ + 1 : k0_;                                            \
-        /* TERMS = 1: this wave's 8 blocks all lie in slot (wave >> 1); the odd  */              \
-        /* slots hold the stage's second k-block                                 */              \
-        const int kk_ = (TERMS == 1 && ((wv >> 1) & 1)) ? k1_ : k0_;                              \
+        const int kk_ = (TERMS == 1 && (wslot & 1)) ? k1_ : k0_;                                  \
+        const __amdgpu_buffer_rsrc_t rs_ = __builtin_amdgcn_make_buffer_rsrc(                     \
+            (void *)(wplane + (size_t)kk_ * planeStride * 8), (short)0, nrec, 0x00020000);        \
         _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                           \
-            __builtin_amdgcn_global_load_lds((const void *)(gsrc[i] + (size_t)kk_ * planeStride + lane_off), \
-                (__attribute__((address_space(3))) void *)(&(arr)[0][0] + loff[i]), 16, 0, 0);    \
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(                                             \
+                rs_, (__attribute__((address_space(3))) void *)(&(arr)[0][0] + loff[i]), 16, voff, \
+                i * 1024, 0, 0);                                                                  \
         }                                                                                         \
     } while (0)
-    // One 32-point k-block kbv from stage arr, operand slots sa (A) and sb (B)
-    // (TERMS = 3: A-lo is slot sa + 1, B-lo slot sb + 1).  The sign flips of
-    // this lane's 8 points (8 lg .. 8 lg + 7) go onto the B fragments; the
-    // MFMAs run term-major, 16 independent accumulators between two updates
-    // of one.
-#define X3_KBLOCK(arr, kbv, sa, sb)                                                               \
+    // The weight signs of this lane's 8 points of a k-block (8 lg .. 8 lg + 7)
+    // are byte lg of the k-block's sign word; the byte is a per-lane load
+    // issued with the DMA one stage ahead, so it has landed by the barrier
+    // (a scalar load at the top of a stage would be waited for with
+    // lgkmcnt(0), which also drains every LDS read).  The flips go onto the
+    // B fragments as a sign-bit XOR.
+    const unsigned char *sgb = reinterpret_cast<const unsigned char *>(sgn) + lg;
+#define X3_SIGNS(s0_, s1_, kbv)                                                                   \
     do {                                                                                          \
-        const unsigned bits_ = (sgn[kbv] >> (8 * lg)) & 0xffu;                                    \
-        u32x4 fl_;                                                                                \
-        fl_.x = ((bits_ & 1u) << 15) | (((bits_ >> 1) & 1u) << 31);                               \
-        fl_.y = (((bits_ >> 2) & 1u) << 15) | (((bits_ >> 3) & 1u) << 31);                        \
-        fl_.z = (((bits_ >> 4) & 1u) << 15) | (((bits_ >> 5) & 1u) << 31);                        \
-        fl_.w = (((bits_ >> 6) & 1u) << 15) | (((bits_ >> 7) & 1u) << 31);                        \
-        bf16x8 ah_[8], al_[8];                                                                    \
-        _Pragma("unroll") for (int a = 0; a < 8; ++a) {                                           \
-            const int p_ = pA + 64 * a;                                                           \
-            ah_[a] = __builtin_bit_cast(bf16x8, (arr)[sa][p_]);                                   \
-            if (TERMS == 3) al_[a] = __builtin_bit_cast(bf16x8, (arr)[(sa) + 1][p_]);             \
+        (s0_) = sgb[4 * (size_t)(kbv)];                                                           \
+        if (TERMS == 1) (s1_) = sgb[4 * ((size_t)(kbv) + 1)];                                     \
+    } while (0)
+#define X3_FLIP(fl_, bits_)                                                                       \
+    do {                                                                                          \
+        const unsigned b_ = (bits_);                                                              \
+        (fl_).x = ((b_ & 1u) << 15) | (((b_ >> 1) & 1u) << 31);                                   \
+        (fl_).y = (((b_ >> 2) & 1u) << 15) | (((b_ >> 3) & 1u) << 31);                            \
+        (fl_).z = (((b_ >> 4) & 1u) << 15) | (((b_ >> 5) & 1u) << 31);                            \
+        (fl_).w = (((b_ >> 6) & 1u) << 15) | (((b_ >> 7) & 1u) << 31);                            \
+    } while (0)
+    // Within a stage the 24 fragment reads are issued in the order of first
+    // use and interleaved with the MFMAs: 12 reads ahead, then one read per
+    // G MFMAs, so the first MFMA waits for the first few reads rather than
+    // for all of them and every later read is issued well before its use.
+#define X3_SCHED(G, nmfma)                                                                        \
+    do {                                                                                          \
+        __builtin_amdgcn_sched_group_barrier(0x100, 12, 0);                                       \
+        _Pragma("unroll") for (int g_ = 0; g_ < 12; ++g_) {                                       \
+            __builtin_amdgcn_sched_group_barrier(0x008, (G), 0);                                  \
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                    \
         }                                                                                         \
-        _Pragma("unroll") for (int half = 0; half < 2; ++half) {                                  \
-            bf16x8 bh_[2], bl_[2];                                                                \
-            _Pragma("unroll") for (int bb = 0; bb < 2; ++bb) {                                    \
-                const int p_ = pB + 64 * (2 * half + bb);                                         \
-                bh_[bb] = lds_frag_xor((arr)[sb][p_], fl_);                                       \
-                if (TERMS == 3) bl_[bb] = lds_frag_xor((arr)[(sb) + 1][p_], fl_);                 \
-            }                                                                                     \
-            _Pragma("unroll") for (int term = 0; term < TERMS; ++term)                            \
-                _Pragma("unroll") for (int a = 0; a < 8; ++a)                                     \
-                    _Pragma("unroll") for (int bb = 0; bb < 2; ++bb) {                            \
-                        const bf16x8 x_ = term == 2 ? al_[a] : ah_[a];                            \
-                        const bf16x8 y_ = term == 1 ? bl_[bb] : bh_[bb];                          \
-                        acc[a][2 * half + bb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(          \
-                            x_, y_, acc[a][2 * half + bb], 0, 0, 0);                              \
-                    }                                                                             \
-        }                                                                                         \
+        __builtin_amdgcn_sched_group_barrier(0x008, (nmfma) - 12 * (G), 0);                       \
     } while (0)
     // LDS positions of this lane's fragment rows: row r = wr 128 + 16 a + li
     // (A) or wc 64 + 16 b + li (B) holds chunk lg at r 4 + (lg ^ ((r >> 1) & 3)),
@@ -754,44 +545,108 @@ __global__ void __launch_bounds__(512, 1)
     int pB = (wc * 64 + li) * 4 + (lg ^ ((li >> 1) & 3));
     asm("" : "+v"(pA));
     asm("" : "+v"(pB));
-    // the k-blocks kbv.. of stage arr (cnt_ of them, at most KS)
-#define X3_STEP(arr, kbv, cnt_)                                                                   \
+    // TERMS = 3: one k-block, slots A-hi 0, A-lo 1, B-hi 2, B-lo 3.  The 8 B
+    // fragments stay in registers and the A fragments stream through: row
+    // block a gets hi.hi, hi.lo, lo.hi on its 4 accumulators (4 independent
+    // MFMAs between two updates of one accumulator), and the reads of row
+    // block a >= 2 are issued 12 and 18 MFMAs ahead of their first use.
+#define X3_STEP3(arr, sbits)                                                                      \
     do {                                                                                          \
-        if (TERMS == 3) {                                                                         \
-            X3_KBLOCK(arr, kbv, 0, 2);                                                            \
-        } else {                                                                                  \
-            X3_KBLOCK(arr, kbv, 0, 2);                                                            \
-            if ((cnt_) > 1) X3_KBLOCK(arr, (kbv) + 1, 1, 3);                                      \
+        u32x4 fl_;                                                                                \
+        X3_FLIP(fl_, sbits);                                                                      \
+        bf16x8 ah_[8], al_[8], bh_[4], bl_[4];                                                    \
+        _Pragma("unroll") for (int b = 0; b < 4; ++b) bh_[b] = lds_frag_xor((arr)[2][pB + 64 * b], fl_); \
+        ah_[0] = __builtin_bit_cast(bf16x8, (arr)[0][pA]);                                        \
+        _Pragma("unroll") for (int b = 0; b < 4; ++b) bl_[b] = lds_frag_xor((arr)[3][pB + 64 * b], fl_); \
+        al_[0] = __builtin_bit_cast(bf16x8, (arr)[1][pA]);                                        \
+        _Pragma("unroll") for (int a = 1; a < 8; ++a) {                                           \
+            ah_[a] = __builtin_bit_cast(bf16x8, (arr)[0][pA + 64 * a]);                           \
+            al_[a] = __builtin_bit_cast(bf16x8, (arr)[1][pA + 64 * a]);                           \
         }                                                                                         \
+        _Pragma("unroll") for (int a = 0; a < 8; ++a)                                             \
+            _Pragma("unroll") for (int term = 0; term < 3; ++term)                                \
+                _Pragma("unroll") for (int b = 0; b < 4; ++b) {                                   \
+                    const bf16x8 x_ = term == 2 ? al_[a] : ah_[a];                                \
+                    const bf16x8 y_ = term == 1 ? bl_[b] : bh_[b];                                \
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x_, y_, acc[a][b], 0, 0, 0); \
+                }                                                                                 \
+        X3_SCHED(6, 96);                                                                          \
     } while (0)
-    // each __syncthreads waits for this wave's DMA into the stage about to be
-    // read (the compiler's vmcnt(0) before s_barrier) and, through the
-    // barrier, for every wave's DMA into it and every wave's reads of the
-    // other stage, which the next DMA then overwrites
-    // The sched_barrier after each DMA keeps its issue at the top of the
-    // stage: left alone, the scheduler sinks the DMA below the stage's MFMAs,
-    // next to the barrier that waits for it, and its latency is exposed.
-    if (kb0 < kb1) X3_DMA(lds0, kb0);
+    // TERMS = 1: two k-blocks, slots A[kb] 0, A[kb + 1] 1, B[kb] 2,
+    // B[kb + 1] 3; k-block kb first for every accumulator
+#define X3_STEP1(arr, sbits0, sbits1)                                                             \
+    do {                                                                                          \
+        u32x4 f0_, f1_;                                                                           \
+        X3_FLIP(f0_, sbits0);                                                                     \
+        X3_FLIP(f1_, sbits1);                                                                     \
+        bf16x8 a0_[8], a1_[8], b0_[4], b1_[4];                                                    \
+        _Pragma("unroll") for (int b = 0; b < 4; ++b) b0_[b] = lds_frag_xor((arr)[2][pB + 64 * b], f0_); \
+        _Pragma("unroll") for (int a = 0; a < 8; ++a)                                             \
+            a0_[a] = __builtin_bit_cast(bf16x8, (arr)[0][pA + 64 * a]);                           \
+        _Pragma("unroll") for (int b = 0; b < 4; ++b) b1_[b] = lds_frag_xor((arr)[3][pB + 64 * b], f1_); \
+        _Pragma("unroll") for (int a = 0; a < 8; ++a)                                             \
+            a1_[a] = __builtin_bit_cast(bf16x8, (arr)[1][pA + 64 * a]);                           \
+        _Pragma("unroll") for (int k_ = 0; k_ < 2; ++k_)                                          \
+            _Pragma("unroll") for (int a = 0; a < 8; ++a)                                         \
+                _Pragma("unroll") for (int b = 0; b < 4; ++b)                                     \
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(                          \
+                        k_ ? a1_[a] : a0_[a], k_ ? b1_[b] : b0_[b], acc[a][b], 0, 0, 0);         \
+        X3_SCHED(3, 64);                                                                          \
+    } while (0)
+#define X3_STEP(arr, s0_, s1_)                                                                    \
+    do {                                                                                          \
+        if constexpr (TERMS == 3) X3_STEP3(arr, s0_);                                             \
+        else X3_STEP1(arr, s0_, s1_);                                                             \
+    } while (0)
+    // Each barrier is preceded by a wait for all of this wave's outstanding
+    // vector memory operations, so that once every wave has passed it, every
+    // wave's DMA into the stage about to be read has landed; the barrier
+    // also orders every wave's reads of the other stage before the next DMA
+    // overwrites it.  The wait is explicit: the compiler's own waits cover
+    // only this wave's reads of its own DMA blocks, and it may leave the last
+    // DMAs of a stage outstanding at the barrier and wait for them after it,
+    // while other waves already read the rows those DMAs write.  The
+    // sched_barrier after each DMA keeps its issue at the top of the stage:
+    // left alone, the scheduler sinks the DMA below the stage's MFMAs, next
+    // to the barrier that waits for it, and its latency is exposed.
+#define X3_BARRIER()                                                                              \
+    do {                                                                                          \
+        __builtin_amdgcn_s_waitcnt(0x0f70); /* vmcnt(0); expcnt, lgkmcnt not waited */           \
+        __syncthreads();                                                                          \
+    } while (0)
+    unsigned sa0 = 0, sb0 = 0, sa1 = 0, sb1 = 0;  // sign bytes of the stages in lds0, lds1
+    if (kb0 < kb1) {
+        X3_DMA(lds0, kb0);
+        X3_SIGNS(sa0, sb0, kb0);
+    }
     int kb = kb0;
     for (; kb + 2 * KS <= kb1; kb += 2 * KS) {
-        __syncthreads();
+        X3_BARRIER();
         X3_DMA(lds1, kb + KS);
+        X3_SIGNS(sa1, sb1, kb + KS);
         __builtin_amdgcn_sched_barrier(0);
-        X3_STEP(lds0, kb, KS);
-        __syncthreads();
-        X3_DMA(lds0, kb + 2 * KS < kb1 ? kb + 2 * KS : kb + KS);
+        X3_STEP(lds0, sa0, sb0);
+        X3_BARRIER();
+        const int nx = kb + 2 * KS < kb1 ? kb + 2 * KS : kb + KS;
+        X3_DMA(lds0, nx);
+        X3_SIGNS(sa0, sb0, nx);
         __builtin_amdgcn_sched_barrier(0);
-        X3_STEP(lds1, kb + KS, KS);
+        X3_STEP(lds1, sa1, sb1);
     }
     // the launcher makes every k-range a multiple of KS, so at most one
     // stage is left (already in lds0)
     if (kb < kb1) {
-        __syncthreads();
-        X3_STEP(lds0, kb, kb1 - kb);
+        X3_BARRIER();
+        X3_STEP(lds0, sa0, sb0);
     }
 #undef X3_DMA
-#undef X3_KBLOCK
+#undef X3_SIGNS
+#undef X3_FLIP
+#undef X3_SCHED
+#undef X3_STEP3
+#undef X3_STEP1
 #undef X3_STEP
+#undef X3_BARRIER
 
     // D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4
     const int oj = lane & 15, oi = 4 * (lane >> 4);
@@ -807,8 +662,8 @@ __global__ void __launch_bounds__(512, 1)
                 const int tr = tr0 + e;
                 const int64_t row = (int64_t)rowA + tr;
                 const float v = acc[a][b][e];
-                if (part) {
-                    part[((size_t)split * ntiles + tile) * (kFT * kFT) + (size_t)tr * kFT + tc] = v;
+                if (out >= 0) {
+                    part[(size_t)out * (kFT * kFT) + (size_t)tr * kFT + tc] = v;
                 } else if (row <= col && col < n) {
                     const size_t idx = (size_t)col * n + row;
                     double h = H[idx], c = Hc[idx];
@@ -860,6 +715,10 @@ struct WorkOrder {
     int nsplit = 0;
     int4 *d = nullptr;
     int count = 0;
+    // unsplit order: the last round's tiles, each split over K into
+    // tail_split items whose FP32 partials k_syrk_x3_fold sums into H
+    int ntail = 0, tail_split = 0;
+    int2 *d_tail = nullptr;
 };
 // one entry per (context, n, nsplit): the gradient estimator alternates
 // between the basis and its per-mode blocks
@@ -903,6 +762,7 @@ void release_fast_tiles(scp3g_ctx *ctx)
     for (auto iw = g_work.begin(); iw != g_work.end();) {
         if (std::get<0>(iw->first) == ctx) {
             if (iw->second.d) hipFree(iw->second.d);
+            if (iw->second.d_tail) hipFree(iw->second.d_tail);
             iw = g_work.erase(iw);
         } else {
             ++iw;
@@ -990,8 +850,8 @@ hipError_t launch_corr_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int64
     return hipGetLastError();
 }
 
-// Dispatch order of the (tile, split) work items.  Consecutive workgroup
-// ids go to the 8 XCDs in turn and each XCD has its own L2.
+// Dispatch order of the work items.  Consecutive workgroup ids go to the 8
+// XCDs in turn and each XCD has its own L2.
 //  * split K (few tiles): the items are listed split-major (the tiles of one
 //    split read the same few row blocks of that K range), the list is cut
 //    into 8 contiguous chunks, and chunk x is dealt to the workgroup ids
@@ -1002,26 +862,37 @@ hipError_t launch_corr_split(scp3g_ctx *ctx, const double *y, int64_t ldy, int64
 //    one per XCD), so that the XCDs run the same super-tile together: each
 //    XCD's L2 holds its sub-block's 6 row blocks, and the Infinity Cache the
 //    16 row blocks that the XCDs share.  (Measured at 20825 states: 3.5%
-//    faster than 4 x 8 super-tiles per XCD chunk.)
+//    faster than 4 x 8 super-tiles per XCD chunk.)  One workgroup fills a
+//    CU, so the tiles run in rounds of ncu; when the last round would be
+//    partly idle (ntiles % ncu = r tiles), those r tiles are split over K
+//    into s = ncu / r items each, which fit one round of 1/s the length,
+//    and their FP32 partials are folded into H afterwards.  (At 20825
+//    states, 3403 tiles on 256 CUs: 13 rounds and a 75-tile one that took
+//    as long as a full round; now 13 rounds and a third.)
 
-static hipError_t work_order(scp3g_ctx *ctx, int64_t n, int nsplit, const int4 **work, int *count)
+static hipError_t work_order(scp3g_ctx *ctx, int64_t n, int nsplit, int ncu, WorkOrder **out)
 {
-    std::lock_guard<std::mutex> guard_(g_fast_lock);
     WorkOrder &wo = g_work[std::make_tuple(ctx, n, nsplit)];
     if (wo.n != n || wo.nsplit != nsplit) {
         if (wo.d) hipFree(wo.d);
+        if (wo.d_tail) hipFree(wo.d_tail);
         wo.d = nullptr;
+        wo.d_tail = nullptr;
         wo.n = -1;
+        wo.ntail = wo.tail_split = 0;
         const int T = (int)((n + kFT - 1) / kFT);
+        if (T > 0xffff) return hipErrorInvalidValue;
         const int ntiles = T * (T + 1) / 2;
+        auto item = [](int i, int j, int sp, int slot) { return make_int4(i | (j << 16), sp, 0, slot); };
         std::vector<int4> v;
+        std::vector<int2> tail;
         v.reserve((size_t)ntiles * nsplit);
         if (nsplit > 1) {
             std::vector<int4> grouped;
             grouped.reserve((size_t)ntiles * nsplit);
             for (int sp = 0; sp < nsplit; ++sp)
                 for (int j = 0; j < T; ++j)
-                    for (int i = 0; i <= j; ++i) grouped.push_back(make_int4(i, j, sp, j * (j + 1) / 2 + i));
+                    for (int i = 0; i <= j; ++i) grouped.push_back(item(i, j, sp, sp * ntiles + j * (j + 1) / 2 + i));
             const int total = (int)grouped.size();
             const int q = total / 8, r = total % 8;
             v.resize((size_t)total);
@@ -1042,8 +913,7 @@ static hipError_t work_order(scp3g_ctx *ctx, int64_t n, int nsplit, const int4 *
                     std::vector<int4> sub[8];
                     for (int j = J0; j < std::min(J0 + S, T); ++j)
                         for (int i = I0; i < std::min(I0 + S, j + 1); ++i)
-                            sub[((i - I0) / RI) * 2 + (j - J0) / CJ].push_back(
-                                make_int4(i, j, 0, j * (j + 1) / 2 + i));
+                            sub[((i - I0) / RI) * 2 + (j - J0) / CJ].push_back(item(i, j, 0, -1));
                     size_t mx = 0;
                     for (auto &q : sub) mx = std::max(mx, q.size());
                     for (size_t k = 0; k < mx; ++k)
@@ -1052,21 +922,64 @@ static hipError_t work_order(scp3g_ctx *ctx, int64_t n, int nsplit, const int4 *
                 }
             std::vector<char> seen((size_t)ntiles, 0);
             for (const int4 &w : v) {
-                if (w.w < 0 || w.w >= ntiles || seen[w.w]) return hipErrorInvalidValue;
-                seen[w.w] = 1;
+                const int i = w.x & 0xffff, j = w.x >> 16;
+                const int c = j * (j + 1) / 2 + i;
+                if (i > j || j >= T || seen[c]) return hipErrorInvalidValue;
+                seen[c] = 1;
+            }
+            if ((int64_t)v.size() != ntiles) return hipErrorInvalidValue;
+            const int r = ntiles % ncu, s = r ? ncu / r : 0;
+            if (ntiles > ncu && r > 0 && s >= 2) {
+                std::vector<int4> last(v.end() - r, v.end());
+                v.resize(v.size() - r);
+                for (int q = 0; q < r; ++q) {
+                    const int i = last[q].x & 0xffff, j = last[q].x >> 16;
+                    tail.push_back(make_int2(i, j));
+                    for (int sp = 0; sp < s; ++sp) v.push_back(item(i, j, sp, sp * r + q));
+                }
+                wo.ntail = r;
+                wo.tail_split = s;
             }
         }
-        if ((int64_t)v.size() != (int64_t)ntiles * nsplit) return hipErrorInvalidValue;
         hipError_t e = hipMalloc(&wo.d, v.size() * sizeof(int4));
         if (e != hipSuccess) return e;
         e = hipMemcpy(wo.d, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice);
         if (e != hipSuccess) return e;
+        if (!tail.empty()) {
+            if ((e = hipMalloc(&wo.d_tail, tail.size() * sizeof(int2))) != hipSuccess) return e;
+            e = hipMemcpy(wo.d_tail, tail.data(), tail.size() * sizeof(int2), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return e;
+        }
         wo.n = n;
         wo.nsplit = nsplit;
         wo.count = (int)v.size();
     }
-    *work = wo.d;
-    *count = wo.count;
+    *out = &wo;
+    return hipSuccess;
+}
+
+static int device_cus(int device)
+{
+    static std::mutex m;
+    static std::map<int, int> cus;
+    std::lock_guard<std::mutex> guard_(m);
+    auto it = cus.find(device);
+    if (it != cus.end()) return it->second;
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c <= 0) c = 256;
+    cus[device] = c;
+    return c;
+}
+
+static hipError_t ensure_part(scp3g_ctx *ctx, size_t need)
+{
+    if (ctx->part_bytes >= need) return hipSuccess;
+    if (ctx->d_part) hipFree(ctx->d_part);
+    ctx->d_part = nullptr;
+    ctx->part_bytes = 0;
+    hipError_t e = hipMalloc(&ctx->d_part, need);
+    if (e != hipSuccess) return e;
+    ctx->part_bytes = need;
     return hipSuccess;
 }
 
@@ -1079,55 +992,46 @@ hipError_t launch_syrk_x3(scp3g_ctx *ctx, int64_t n, const unsigned short *Xh,
     hipError_t e = fast_tiles(ctx, n, &tiles, &ntiles);
     if (e != hipSuccess) return e;
     const int nkb = (int)(K / 32);
+    if (terms == 1 && (nkb & 1)) return hipErrorInvalidValue;  // K is a multiple of 64
     int nsplit = 1;
     while (ntiles * nsplit < 512 && nkb / (nsplit * 2) >= 8) nsplit *= 2;
     int kbs = (nkb + nsplit - 1) / nsplit;
     if (terms == 1) kbs += kbs & 1;  // k-ranges of whole two-block stages
     nsplit = (nkb + kbs - 1) / kbs;
-    if (terms == 1 && (nkb & 1)) return hipErrorInvalidValue;  // K is a multiple of 64
-    // the 32x32 variant stages a split's sign words in LDS
-    if (ctx->x3_variant == 2 && kbs > kMaxSignWords) return hipErrorInvalidValue;
+    WorkOrder *wo;
+    {
+        std::lock_guard<std::mutex> guard_(g_fast_lock);
+        if ((e = work_order(ctx, n, nsplit, device_cus(ctx->device), &wo)) != hipSuccess) return e;
+    }
+    // the split-K partials, or those of the last round's split tiles
+    const int fold_tiles = nsplit > 1 ? ntiles : wo->ntail;
+    const int fold_split = nsplit > 1 ? nsplit : wo->tail_split;
+    const int2 *fold_list = nsplit > 1 ? tiles : wo->d_tail;
+    if (nsplit == 1 && fold_tiles) {
+        kbs = (nkb + fold_split - 1) / fold_split;
+        kbs += kbs & 1;
+    }
     float *part = nullptr;
-    if (nsplit > 1) {
-        const size_t need = (size_t)nsplit * ntiles * kFT * kFT * sizeof(float);
-        if (ctx->part_bytes < need) {
-            if (ctx->d_part) hipFree(ctx->d_part);
-            ctx->d_part = nullptr;
-            ctx->part_bytes = 0;
-            if ((e = hipMalloc(&ctx->d_part, need)) != hipSuccess) return e;
-            ctx->part_bytes = need;
-        }
+    if (fold_tiles) {
+        if ((e = ensure_part(ctx, (size_t)fold_split * fold_tiles * kFT * kFT * sizeof(float))) != hipSuccess)
+            return e;
         part = reinterpret_cast<float *>(ctx->d_part);
     }
     const int64_t rows_pad = fast_rows_pad(n);
-    const int4 *work;
-    int nwork;
-    if ((e = work_order(ctx, n, nsplit, &work, &nwork)) != hipSuccess) return e;
     const uint4 *xh = reinterpret_cast<const uint4 *>(Xh), *xl = reinterpret_cast<const uint4 *>(Xl);
-    const bool m32 = ctx->x3_variant == 2;
     hipEvent_t ev;
     timer_begin(ctx, "syrk_mma", &ev);
-    if (m32 && terms == 1)
-        hipLaunchKernelGGL(k_syrk_x3m<1>, dim3((unsigned)nwork), dim3(256), 0, ctx->stream, xh, xl, sgn, rows_pad,
-                           work, ntiles, kbs, nkb, n, H, Hc, part);
-    else if (m32)
-        hipLaunchKernelGGL(k_syrk_x3m<3>, dim3((unsigned)nwork), dim3(256), 0, ctx->stream, xh, xl, sgn, rows_pad,
-                           work, ntiles, kbs, nkb, n, H, Hc, part);
-    else if (terms == 1)
-        hipLaunchKernelGGL(k_syrk_x3<1>, dim3((unsigned)nwork), dim3(512), 0, ctx->stream, xh, xl, sgn, rows_pad,
-                           work, ntiles, kbs, nkb, n, H, Hc, part);
+    if (terms == 1)
+        hipLaunchKernelGGL(k_syrk_x3<1>, dim3((unsigned)wo->count), dim3(512), 0, ctx->stream, xh, xl, sgn,
+                           rows_pad, wo->d, kbs, nkb, n, H, Hc, part);
     else
-        hipLaunchKernelGGL(k_syrk_x3<3>, dim3((unsigned)nwork), dim3(512), 0, ctx->stream, xh, xl, sgn, rows_pad,
-                           work, ntiles, kbs, nkb, n, H, Hc, part);
+        hipLaunchKernelGGL(k_syrk_x3<3>, dim3((unsigned)wo->count), dim3(512), 0, ctx->stream, xh, xl, sgn,
+                           rows_pad, wo->d, kbs, nkb, n, H, Hc, part);
     timer_end(ctx, "syrk_mma", ev);
     e = hipGetLastError();
-    if (e != hipSuccess || !part) return e;
-    if (m32)
-        hipLaunchKernelGGL(k_syrk_x3m_fold, dim3((unsigned)ntiles, kFT * kFT / 256), dim3(256), 0, ctx->stream,
-                           part, tiles, ntiles, nsplit, n, H, Hc);
-    else
-        hipLaunchKernelGGL(k_syrk_x3_fold, dim3((unsigned)ntiles, kFT * kFT / 256), dim3(256), 0, ctx->stream,
-                           part, tiles, ntiles, nsplit, n, H, Hc);
+    if (e != hipSuccess || !fold_tiles) return e;
+    hipLaunchKernelGGL(k_syrk_x3_fold, dim3((unsigned)fold_tiles, kFT * kFT / 256), dim3(256), 0, ctx->stream,
+                       part, fold_list, fold_tiles, fold_split, n, H, Hc);
     return hipGetLastError();
 }
 

@@ -292,13 +292,12 @@ def test_mixed_matches_fp64_within_budget(ctx, hexamer):
 
 @pytest.mark.parametrize("n2,n3", [(48, 0), (48, 12)])
 def test_mixed_kernels_agree(ctx, hexamer, n2, n3):
-    """The signed 256-tile split-BF16 SYRKs (16x16x32 and 32x32x16 MFMA
-    variants) and the 64-tile one give the same H to FP32 accumulation
-    accuracy, and all stay close to the FP64 form."""
+    """The signed 256-tile split-BF16 SYRK and the 64-tile one give the same
+    H to FP32 accumulation accuracy, and both stay close to the FP64 form."""
     P = 3 * 4096 + 500
     out = {}
     for name, mode, fast in (("fp64", scp3_amd.FP64, 1), ("slow", scp3_amd.MIXED, 0),
-                             ("fast", scp3_amd.MIXED, 1), ("m32", scp3_amd.MIXED, 2)):
+                             ("fast", scp3_amd.MIXED, 1)):
         ctx.set_system(hexamer["x0"], hexamer["omega"], hexamer["MUa"])
         ctx.set_basis(np.arange(48, dtype=np.int32), n2, n3)
         ctx.set_pes("qtip4pf")
@@ -310,7 +309,7 @@ def test_mixed_kernels_agree(ctx, hexamer, n2, n3):
         out[name] = ctx.h_download()
     ctx.set_mixed_kernel(1)
     scale = np.abs(out["fp64"]).max()
-    for k in ("fast", "m32"):
+    for k in ("fast",):
         d_fs = np.abs(out[k] - out["slow"]).max() / scale
         d_f64 = np.abs(out[k] - out["fp64"]).max() / scale
         print("n=%d %s: |-slow| %.2e  |-fp64| %.2e (relative to max|H|)" % (out["fp64"].shape[0], k, d_fs, d_f64))
