@@ -1,28 +1,33 @@
 """Benchmark: Sobol points/s into H and time to the lowest levels.
 
-Default workload (preset c2): water hexamer relaxed under q-TIP4P/F (6 H2O,
-48 modes), basis of total quanta <= 2 (1225 states), 2^20 Sobol points
-(skip 2^30) in 524288-point tiles (the automatic tile).  One step = one
-full pass of the hot path over the 2^20 points (sample -> map -> potential -> basis -> SYRK into the
-FP64 running H) followed by the lowest 10 levels on the GPU (the K timed
+Default workload (preset c3): the water hexamer relaxed under TTM3-F (6
+H2O, 48 modes), basis of total quanta <= 3 (20825 states), 2^21 scrambled
+Sobol points (seed 0, skip 2^30) per step in 32768-point tiles (the
+automatic tile).  One step = one full pass of the hot path over the 2^21
+points (sample -> map -> TTM3-F -> weight -> basis -> SYRK into the FP64
+running H) followed by the lowest 10 levels on the GPU (the K timed
 accumulate passes run first, with per-kernel event timers, then K timed
-eigensolves of the last pass's H without them).  Both GPU forms
-run: the plain FP64 form and the mixed form (FP32 potential and basis,
-3-term split-BF16 MFMA SYRK folded into FP64 H).  `value` is the mixed
-form's points/s into H; the FP64 form, the time-to-levels of both and the
-level difference are reported beside it.  The CPU baseline is the FP64
-oracle path (Sobol/BSM, map, serial potential loop, get_bra, OpenBLAS
-dsyr2k) on a bounded prefix of the same points.
+eigensolves of the last pass's H without them).  Three GPU forms run: the
+mixed form (FP32 potential and basis, 3-term split-BF16 MFMA SYRK folded
+into FP64 H; `value` is its points/s into H), the plain FP64 form (at most
+2 timed steps) and the MLP control variate (2^24 surrogate + 2^23
+true-potential points, one timed step, after its training).  Beside them:
+the level difference and H error budget of mixed against FP64 on the same
+points, each form's time to the lowest 10 levels within 0.5 cm^-1 (this
+run's rates x the points the committed convergence record needs), the
+mixed SYRK's roofline, and the CPU baseline: the -G path as SCP3 runs it
+(the reference's potential in a serial point loop, get_bra under OpenMP,
+OpenBLAS dsyr2k) on a bounded prefix of the same points.
 
-Other presets: c1 (dimer, 91 states, 2^16 points), c3 (hexamer under
-TTM3-F, 20825 states, 2^21 points, 32768-point tiles), c4 (c3 with the MLP
-control variate: 2^24 surrogate + 2^19 true points against plain FP64 at
-2^21), c5 (20-water cluster under TTM3-F, 174 modes, 15400 states, MLP
-174-512-512-512-1 on 2^19 pairs, 2^24 surrogate + 2^20 true points against
-plain FP64 at 2^20, lowest 20 levels); c3g (c3 with the gradient
-estimator, Stein form, 2^20 points, FP64 against mixed); scp1 / scp1q
-(SCP1's Gaussian averages: hexamer under TTM3-F at 2^18 points / under
-q-TIP4P/F at 2^20 points, one step = one SCP1 iteration's averages).
+Other presets: c1 (dimer, 91 states, 2^16 points), c2 (hexamer under
+q-TIP4P/F, 1225 states, 2^20 points, FP64 and mixed), c4 (c3 with the MLP
+control variate as configured: 2^24 surrogate + 2^19 true points against
+plain FP64 at 2^21), c5 (20-water cluster under TTM3-F, 174 modes, 15400
+states, MLP 174-512-512-512-1 on 2^19 pairs, 2^24 surrogate + 2^20 true
+points against plain FP64 at 2^20, lowest 20 levels); c3g (c3 with the
+gradient estimator, Stein form, 2^20 points, FP64 against mixed); scp1 /
+scp1q (SCP1's Gaussian averages: hexamer under TTM3-F at 2^18 points /
+under q-TIP4P/F at 2^20 points, one step = one SCP1 iteration's averages).
 
 Forms: fp64, mixed, cv (above) and grad / stein: the gradient (Stein-
 identity) estimator, the reference's default without -G (grad: the
@@ -31,7 +36,7 @@ identity), in FP64, or with the rank-2 term on split-BF16 MFMA
 (grad_mixed, stein_mixed), e.g. --preset c3 --forms stein,stein_mixed.
 The JSON line's main form is mixed, else cv, else the last form listed.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--preset c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--preset c3]
 """
 import argparse
 import json
@@ -579,7 +584,7 @@ def main():
             print("bench: form %s accumulate %.4f s, levels %.4f s" % (f, res[f]["t_acc"], res[f]["t_lev"]),
                   file=sys.stderr, flush=True)
     n = res[forms[0]]["n"]
-    main_form = forms[0]
+    main_form = "mixed" if "mixed" in forms else ("cv" if "cv" in forms else forms[-1])
     r = res[main_form]
     # dominant kernel: the SYRK's MFMA kernel (split-K partial fold timed
     # apart); algorithmic flops of a symmetric rank-K update: n(n+1) per point
