@@ -98,7 +98,7 @@ def build_cli(verbose=False):
         if os.path.lexists(link):
             os.remove(link)
         os.symlink(real, link)
-        extra = ["-DSCP3G_HAVE_HDF5", "-I" + HDF5_INC, real, "-Wl,-rpath,$ORIGIN/hdf5"]
+        extra = ["-DSCP3G_HAVE_HDF5", "-I" + HDF5_INC, real, "-lz", "-Wl,-rpath,$ORIGIN/hdf5"]
     for src, exe in (("scp3_main.cpp", "SCP3"), ("eigensolver_main.cpp", "eigensolver"), ("scp1_main.cpp", "SCP1")):
         cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-pthread", os.path.join(CSRC, src), "-o", os.path.join(BIN, exe),
                "-L" + HERE, "-lscp3g", "-Wl,-rpath,$ORIGIN/.."] + extra

@@ -26,15 +26,15 @@ def writer(tmp_path_factory):
         pytest.skip("h5dump is absent")
     libs = [f for f in os.listdir(HDF5_LIB_DIR) if f.startswith("libhdf5.so")]
     exe = str(tmp_path_factory.mktemp("h5") / "h5_layout")
-    r = subprocess.run(["g++", "-O1", "-std=c++17", "-DSCP3G_HAVE_HDF5", "-I" + HDF5_INC,
+    r = subprocess.run(["g++", "-O1", "-std=c++17", "-pthread", "-DSCP3G_HAVE_HDF5", "-I" + HDF5_INC,
                         os.path.join(ROOT, "tests", "h5_layout_main.cpp"), "-o", exe,
-                        os.path.join(HDF5_LIB_DIR, libs[0]), "-Wl,-rpath," + HDF5_LIB_DIR],
+                        os.path.join(HDF5_LIB_DIR, libs[0]), "-lz", "-Wl,-rpath," + HDF5_LIB_DIR],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return exe
 
 
-@pytest.mark.parametrize("n", [91, 300, 1225])
+@pytest.mark.parametrize("n", [91, 300, 1225, 1500])
 def test_sm_h5_layout(writer, tmp_path, n):
     path = str(tmp_path / "sM_0000001.h5")
     r = subprocess.run([writer, str(n), path], capture_output=True, text=True)
