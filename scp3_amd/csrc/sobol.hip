@@ -94,48 +94,63 @@ __constant__ double kBsmC[9] = {0.3374754822726147, 0.9761690190917186, 0.160797
                                 0.0276438810333863, 0.0038405729373609, 0.0003951896511919,
                                 0.0000321767881768, 0.0000002888167364, 0.0000003960315187};
 
-// Same operation order as beasley_springer_moro.cpp:25-44, no contraction.
-__device__ __forceinline__ double bsm(double u)
+// Same operation order as beasley_springer_moro.cpp:25-44, no contraction:
+// the central branch (|u - 1/2| < 0.42) and the tail branch as two
+// functions, so that the kernel can run the tail values of a wave together.
+__device__ __forceinline__ double bsm_central(double yj)
+{
+#pragma clang fp contract(off)
+    const double r = yj * yj;
+    return yj * (((kBsmA[3] * r + kBsmA[2]) * r + kBsmA[1]) * r + kBsmA[0]) /
+           ((((kBsmB[3] * r + kBsmB[2]) * r + kBsmB[1]) * r + kBsmB[0]) * r + 1);
+}
+
+__device__ __forceinline__ double bsm_tail(double u)
 {
 #pragma clang fp contract(off)
     const double yj = u - 0.5;
-    double x, r;
-    if (fabs(yj) < 0.42) {
-        r = yj * yj;
-        x = yj * (((kBsmA[3] * r + kBsmA[2]) * r + kBsmA[1]) * r + kBsmA[0]) /
-            ((((kBsmB[3] * r + kBsmB[2]) * r + kBsmB[1]) * r + kBsmB[0]) * r + 1);
-    } else {
-        // one pair of logs per lane: the operand is chosen first
-        const double q = (yj > 0) ? 1 - u : u;
-        r = log(-log(q));
-        x = kBsmC[0] +
-            r * (kBsmC[1] +
-                 r * (kBsmC[2] +
-                      r * (kBsmC[3] +
-                           r * (kBsmC[4] +
-                                r * (kBsmC[5] + r * (kBsmC[6] + r * (kBsmC[7] + r * kBsmC[8])))))));
-        if (yj < 0) x = -x;
-    }
+    // one pair of logs per lane: the operand is chosen first
+    const double q = (yj > 0) ? 1 - u : u;
+    const double r = log(-log(q));
+    double x = kBsmC[0] +
+               r * (kBsmC[1] +
+                    r * (kBsmC[2] +
+                         r * (kBsmC[3] +
+                              r * (kBsmC[4] +
+                                   r * (kBsmC[5] + r * (kBsmC[6] + r * (kBsmC[7] + r * kBsmC[8])))))));
+    if (yj < 0) x = -x;
     return x;
 }
 
 // grid: x = groups of (waves per block * 64 * PPT) points, y = dimension;
 // PPT points per lane (strided by 64): each lane pays ~log2(seed) XORs for
 // its starting state, so large tiles take 16 points per lane and small ones
-// 4, to keep enough waves in flight (launch_sobol)
+// 4, to keep enough waves in flight (launch_sobol).
+//
+// The tail branch of the inverse normal (two FP64 logarithms and a 9-term
+// polynomial) costs several times the central one, and about 16% of the
+// values take it; a wave that branched per value would run it for nearly
+// every value (some lane almost always needs it).  So each lane writes its
+// central values at once and appends its tail values to a queue of its
+// wave in LDS (ballot + prefix count), and the wave then runs the tail
+// branch over the queue, 64 values at a time.  The queue is flushed before
+// it could overflow (at most 64 entries join per point step).
 template <int PPT>
 __global__ void __launch_bounds__(256)
     k_sobol(const uint64_t *__restrict__ dirs, const uint64_t *__restrict__ shift, int64_t skip,
             int64_t npts, int halve, double *__restrict__ y, int64_t ldy, uint64_t *__restrict__ ints)
 {
 #pragma clang fp contract(off)
+    constexpr int QCAP = 256;  // queue entries per wave
+    __shared__ double qu[4][QCAP];
+    __shared__ int qi[4][QCAP];
     const int d = blockIdx.y;
     const uint64_t *__restrict__ v = dirs + (size_t)d * kSobolCols;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int64_t i0 =
-        ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * (64 * PPT) + lane;
-    if (i0 >= npts) return;
+    const int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * (64 * PPT);
+    if (base >= npts) return;  // wave-uniform
+    const int64_t i0 = base + lane;
     uint64_t s = (uint64_t)(skip + i0);
     uint64_t g = s ^ (s >> 1);
     uint64_t st = shift ? shift[d] : 0;
@@ -146,18 +161,43 @@ __global__ void __launch_bounds__(256)
     const uint64_t v5 = v[5];
     double *__restrict__ yd = y + (size_t)d * ldy;
     uint64_t *__restrict__ id = ints ? ints + (size_t)d * ldy : nullptr;
+    double *__restrict__ wq = qu[wave];
+    int *__restrict__ wi = qi[wave];
+    int qn = 0;  // queue length (wave-uniform)
+    auto flush = [&]() {
+        for (int j = lane; j < qn; j += 64) {
+            double x = bsm_tail(wq[j]);
+            if (halve) x = x / 1.4142135623730951;  // by /= sqrt(2.0), SCP3.cpp:201
+            yd[base + wi[j]] = x;
+        }
+        qn = 0;
+    };
     for (int k = 0; k < PPT; ++k) {
         const int64_t i = i0 + 64 * k;
-        if (i >= npts) break;
+        const bool live = i < npts;
         const double u = (double)st * 0x1p-62;  // recipd, sobol.cpp:27780,27846
-        double x = bsm(u);
-        if (halve) x = x / 1.4142135623730951;  // by /= sqrt(2.0), SCP3.cpp:201
-        yd[i] = x;
-        if (id) id[i] = st;
+        const double yj = u - 0.5;
+        const bool tail = live && !(fabs(yj) < 0.42);
+        if (live && !tail) {
+            double x = bsm_central(yj);
+            if (halve) x = x / 1.4142135623730951;
+            yd[i] = x;
+        }
+        if (live && id) id[i] = st;
+        const uint64_t m = __builtin_amdgcn_ballot_w64(tail);
+        if (tail) {
+            const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            wq[pos] = u;
+            wi[pos] = 64 * k + lane;
+        }
+        qn += __builtin_popcountll(m);
+        if (qn > QCAP - 64) flush();
         const int c = __builtin_ctzll(~(s >> 6));
         st ^= v5 ^ v[6 + c];
         s += 64;
     }
+    flush();
 }
 
 hipError_t launch_sobol(scp3g_ctx *ctx, const uint64_t *dirs, int dims, int64_t skip,
