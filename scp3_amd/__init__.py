@@ -221,7 +221,7 @@ class Context:
             self.check(self.L.scp3g_set_scramble(self._h, 1, int(seed)))
 
     def set_mixed_kernel(self, kernel):
-        """0: 64 x 64 direct-load SYRKs; 1, 2: LDS-tiled SYRKs (see scp3g.h)"""
+        """0: 64 x 64 direct-load SYRKs; 1: LDS-tiled SYRKs (see scp3g.h)"""
         self.check(self.L.scp3g_set_mixed_kernel(self._h, int(kernel)))
 
     # whole path ---------------------------------------------------------
