@@ -94,6 +94,8 @@ def parse():
                     help="points per CPU baseline block (the reference's -B, SCP3.cpp:58)")
     ap.add_argument("--converge-summary", nargs="+", default=None,
                     help="JSON-line outputs of --converge runs: write the convergence record of the preset")
+    ap.add_argument("--merge-record", action="store_true",
+                    help="with --converge-summary: keep the committed record's checkpoints and add the new ones")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cv-terms", type=int, default=1)
     ap.add_argument("--scramble", type=int, default=0, help="seed of the affine digit scramble of the Sobol points")
@@ -437,36 +439,59 @@ def convergence_path(pr, n):
     return os.path.join(CONV_DIR, "convergence_%s_%dstates.json" % (pr["config"], n))
 
 
-def write_convergence(pr, files, n):
+def write_convergence(pr, files, n, merge=False):
     """Convergence record of a workload from --converge outputs: the
     reference levels (mean over scramble seeds of the last checkpoint of the
     stein_mixed runs, with the standard error of that mean) and, for each
     -G form, every checkpoint's largest level deviation from them and the
-    first checkpoint from which all levels stay within 0.5 cm^-1."""
+    first checkpoint from which all levels stay within 0.5 cm^-1 (over
+    every seed's checkpoints at and beyond it).  merge: the committed
+    record's checkpoints (levels = reference + deviation) and, when the
+    outputs hold no stein_mixed run, its reference are kept and the new
+    checkpoints added."""
     rows, seen = [], set()
+
+    def add(r):
+        key = (r.get("form"), r.get("seed"), r.get("points"), r.get("points_t"))
+        # a run continued from an earlier one repeats that run's last checkpoint
+        if "points" in r and key not in seen:
+            seen.add(key)
+            rows.append(r)
+
     for fn in files:
         for line in open(fn):
-            if not line.strip().startswith("{"):
-                continue
-            r = json.loads(line)
-            key = (r.get("form"), r.get("seed"), r.get("points"), r.get("points_t"))
-            # a run continued from an earlier one repeats that run's last checkpoint
-            if "points" in r and key not in seen:
-                seen.add(key)
-                rows.append(r)
+            if line.strip().startswith("{"):
+                add(json.loads(line))
+    old = None
+    if merge and os.path.exists(convergence_path(pr, n)):
+        old = json.load(open(convergence_path(pr, n)))
+        oref = np.array(old["reference"]["levels_cm1"])
+        for form, fr in old["forms"].items():
+            for t in fr["checkpoints"]:
+                add({"form": form, "seed": t["seed"], "points": t["points"], "points_t": t["points_t"],
+                     "accumulate_s": t["accumulate_s"], "train_s": t.get("train_s", 0.0),
+                     "levels_cm1": [float(v) for v in oref + np.array(t["dev_cm1"])]})
     st = [r for r in rows if r["form"] == "stein_mixed"]
-    pmax = max(r["points"] for r in st)
-    L = np.array([r["levels_cm1"] for r in st if r["points"] == pmax])
-    ref = L.mean(0)
-    sem = L.std(0, ddof=1) / np.sqrt(L.shape[0]) if L.shape[0] > 1 else np.zeros_like(ref)
+    if st:
+        pmax = max(r["points"] for r in st)
+        L = np.array([r["levels_cm1"] for r in st if r["points"] == pmax])
+        ref = L.mean(0)
+        sem = L.std(0, ddof=1) / np.sqrt(L.shape[0]) if L.shape[0] > 1 else np.zeros_like(ref)
+        reference = {"estimator": "gradient (Stein), mixed form", "points": pmax,
+                     "seeds": sorted(r["seed"] for r in st if r["points"] == pmax),
+                     "levels_cm1": [round(float(v), 4) for v in ref],
+                     "sem_cm1": [round(float(v), 4) for v in sem]}
+    else:
+        reference = old["reference"]
+        ref = np.array(reference["levels_cm1"])
     rec = {"workload": "%s, %d states, scrambled Sobol points (skip 2^30)" % (pr["config"], n),
-           "reference": {"estimator": "gradient (Stein), mixed form", "points": pmax,
-                         "seeds": sorted(r["seed"] for r in st if r["points"] == pmax),
-                         "levels_cm1": [round(float(v), 4) for v in ref],
-                         "sem_cm1": [round(float(v), 4) for v in sem]},
+           "reference": reference,
            "forms": {}}
+    # the control variate's checkpoints: those of the preset's correction-term size only
+    cv_pt = (1 << pr["cv_t"]) if pr.get("cv_t") is not None else None
+    rows = [r for r in rows if r["form"] != "cv" or cv_pt is None or r.get("points_t") == cv_pt]
     for form in sorted({r["form"] for r in rows} - {"stein_mixed"}):
-        cps = sorted((r for r in rows if r["form"] == form), key=lambda r: r["points"])
+        cps = sorted((r for r in rows if r["form"] == form), key=lambda r: (r["points"], r["seed"]))
         table = []
         for r in cps:
             dev = float(np.abs(np.array(r["levels_cm1"]) - ref).max())
@@ -474,14 +499,17 @@ def write_convergence(pr, files, n):
                           "accumulate_s": r["accumulate_s"], "train_s": r.get("train_s", 0.0),
                           "max_dev_cm1": round(dev, 3),
                           "dev_cm1": [round(float(v), 3) for v in np.array(r["levels_cm1"]) - ref]})
+        # the smallest point count from which every checkpoint, of every
+        # seed, has all levels within 0.5 cm^-1
         req = None
-        for q in range(len(table)):
-            if all(t["max_dev_cm1"] <= 0.5 for t in table[q:]):
-                req = table[q]["points"]
+        for P in sorted({t["points"] for t in table}):
+            if all(t["max_dev_cm1"] <= 0.5 for t in table if t["points"] >= P):
+                req = P
                 break
         rec["forms"][form] = {"checkpoints": table, "points_required": req,
                               "points_t": table[0]["points_t"] if table else None,
-                              "dev_at_required_cm1": next((t["dev_cm1"] for t in table if t["points"] == req), None)}
+                              "dev_at_required_cm1": {str(t["seed"]): t["dev_cm1"] for t in table
+                                                      if t["points"] == req} if req else None}
     path = convergence_path(pr, n)
     with open(path, "w") as f:
         json.dump(rec, f, indent=1)
@@ -510,14 +538,16 @@ def time_to_half_wavenumber(pr, res, n):
     cvr = rec["forms"].get("cv", {}).get("points_required")
     if "cv" in res and cvr and "mixed" in res:
         rr = res["cv"]
-        Pt = 1 << pr["cv_t"]
-        t_t = Pt / (res["mixed"]["points"] / res["mixed"]["t_acc"])
-        rate_nn = (1 << pr["cv_nn"]) / max(rr["t_acc"] - t_t, 1e-9)
+        # this run's step: 2^cv_nn surrogate points plus a correction term
+        # over 2^cv_t true points, which run at the mixed form's rate
+        rate_t = res["mixed"]["points"] / res["mixed"]["t_acc"]
+        rate_nn = (1 << pr["cv_nn"]) / max(rr["t_acc"] - (1 << pr["cv_t"]) / rate_t, 1e-9)
+        # the record's correction term may have had more true points
+        Pt = rec["forms"]["cv"].get("points_t") or (1 << pr["cv_t"])
         out["cv"] = {"points": cvr, "points_t": Pt, "surrogate_points_per_s": rate_nn,
-                     "train_s": rr["train_s"], "s": rr["train_s"] + cvr / rate_nn + t_t + rr["t_lev"],
+                     "train_s": rr["train_s"],
+                     "s": rr["train_s"] + cvr / rate_nn + Pt / rate_t + rr["t_lev"],
                      "level_error_cm1": rec["forms"]["cv"]["dev_at_required_cm1"]}
-        if rec["forms"]["cv"].get("points_t") not in (None, Pt):
-            out["cv"]["note"] = "the record's correction term had %d true points" % rec["forms"]["cv"]["points_t"]
     if "fp64" in out and "cv" in out:
         out["speedup_cv_vs_fp64"] = out["fp64"]["s"] / out["cv"]["s"]
     if "fp64" in out and "mixed" in out:
@@ -558,9 +588,11 @@ def main():
     if a.mlp_steps is not None:
         pr["mlp_steps"] = a.mlp_steps
     if a.converge_summary:
-        from oracle import oracle
-        n = oracle.basis_size(sysd["modes"].size, sysd["nmodes2"], sysd["nmodes3"])
-        write_convergence(pr, a.converge_summary, n)
+        # states: ground, singles of the selected modes, doubles of the first
+        # n2 and triples of the first n3 (ScaledMatrixElements.cpp:81-99)
+        m2, m3 = sysd["nmodes2"], sysd["nmodes3"]
+        n = 1 + sysd["modes"].size + m2 * (m2 + 1) // 2 + m3 * (m3 + 1) * (m3 + 2) // 6
+        write_convergence(pr, a.converge_summary, n, merge=a.merge_record)
         return
     if a.converge:
         converge(a, pr, sysd)
