@@ -539,14 +539,18 @@ def time_to_half_wavenumber(pr, res, n):
     if "cv" in res and cvr and "mixed" in res:
         rr = res["cv"]
         # this run's step: 2^cv_nn surrogate points plus a correction term
-        # over 2^cv_t true points, which run at the mixed form's rate
+        # over 2^cv_t true points, each a mixed-form point and, when the
+        # surrogate takes one product, a surrogate point (the correction
+        # subtracts w_nn with that same product)
         rate_t = res["mixed"]["points"] / res["mixed"]["t_acc"]
-        rate_nn = (1 << pr["cv_nn"]) / max(rr["t_acc"] - (1 << pr["cv_t"]) / rate_t, 1e-9)
+        both = pr.get("cv_terms", 1) == 1
+        Pt_run = 1 << pr["cv_t"]
+        rate_nn = ((1 << pr["cv_nn"]) + (Pt_run if both else 0)) / max(rr["t_acc"] - Pt_run / rate_t, 1e-9)
         # the record's correction term may have had more true points
-        Pt = rec["forms"]["cv"].get("points_t") or (1 << pr["cv_t"])
+        Pt = rec["forms"]["cv"].get("points_t") or Pt_run
         out["cv"] = {"points": cvr, "points_t": Pt, "surrogate_points_per_s": rate_nn,
                      "train_s": rr["train_s"],
-                     "s": rr["train_s"] + cvr / rate_nn + Pt / rate_t + rr["t_lev"],
+                     "s": rr["train_s"] + (cvr + (Pt if both else 0)) / rate_nn + Pt / rate_t + rr["t_lev"],
                      "level_error_cm1": rec["forms"]["cv"]["dev_at_required_cm1"]}
     if "fp64" in out and "cv" in out:
         out["speedup_cv_vs_fp64"] = out["fp64"]["s"] / out["cv"]["s"]
@@ -587,6 +591,7 @@ def main():
         pr["cv_nn"] = a.cv_nn
     if a.mlp_steps is not None:
         pr["mlp_steps"] = a.mlp_steps
+    pr["cv_terms"] = a.cv_terms
     if a.converge_summary:
         # states: ground, singles of the selected modes, doubles of the first
         # n2 and triples of the first n3 (ScaledMatrixElements.cpp:81-99)

@@ -240,7 +240,10 @@ int scp3g_mlp_eval(scp3g_ctx *ctx, const double *y_dev, int64_t ldy, int64_t npt
 int scp3g_accumulate_cv(scp3g_ctx *ctx, int64_t skip_nn, int64_t P_nn, int64_t skip_t,
                         int64_t P_t);
 /* mixed form: split-BF16 terms used for the surrogate term of the control
- * variate, 3 (default, hi.hi + hi.lo + lo.hi) or 1 (hi.hi) */
+ * variate, 3 (default, hi.hi + hi.lo + lo.hi) or 1 (hi.hi).  With 1 the
+ * correction term takes bV with three products less w_nn with the same one
+ * product, so the one-product rounding of the surrogate term cancels in
+ * expectation (no reference counterpart) */
 int scp3g_set_cv_terms(scp3g_ctx *ctx, int terms);
 /* the surrogate term alone: H += sum B B^T w_nn over Sobol seeds
  * skip..skip+npts-1, normaliser += npts (QMC with V_nn in place of V, the

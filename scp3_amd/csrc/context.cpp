@@ -568,7 +568,11 @@ int scp3g_reset(scp3g_ctx *ctx)
 // 2: control-variate correction bV - w_nn; 3: plain weights from energies
 // the caller put in ctx->d_E; 5: the correction bV - w_nn as a sum of its
 // own.  Modes 1, 2 and 5 multiply the weights by `scale`; the normaliser
-// grows by np for modes 0, 3 and 5, np*scale for 1.
+// grows by np for modes 0, 3 and 5, np*scale for 1.  When the surrogate
+// term takes one BF16 product (cv_one_term), the correction is bV with
+// three products less w_nn with that same one product, so that the
+// surrogate's rounding cancels in expectation and the estimator keeps the
+// three-product form's precision.
 }  // extern "C"
 
 // The gradient estimator's tile (SCP3.cpp:205-231 with use_gradient):
@@ -672,6 +676,8 @@ int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
 {
     const int64_t T = ctx->work_tile;
     const int fp32 = ctx->precision == SCP3G_MIXED;
+    // a correction term matched to a one-product surrogate term
+    const bool matched = (wmode == 2 || wmode == 5) && fp32 && ctx->mixed_fast && ctx->cv_one_term;
     hipEvent_t ev;
     int rc;
     if (ctx->estimator != SCP3G_EST_ENERGY) {
@@ -703,7 +709,12 @@ int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
     timer_end(ctx, "pes", ev);
     if (rc) return rc;
 weights:
-    if (wmode == 2 || wmode == 5) {
+    if (matched) {
+        // first the true weights alone, scaled
+        timer_begin(ctx, "weight", &ev);
+        rc = check_hip(ctx, launch_weight(ctx, ctx->d_E, ctx->d_y, T, np, ctx->d_w, scale), "weight");
+        timer_end(ctx, "weight", ev);
+    } else if (wmode == 2 || wmode == 5) {
         timer_begin(ctx, "mlp", &ev);
         rc = check_hip(ctx, launch_mlp_weights(ctx, ctx->d_y, T, np, ctx->d_E, 1, scale, ctx->d_w), "mlp");
         timer_end(ctx, "mlp", ev);
@@ -721,16 +732,26 @@ basis:
         unsigned short *Xh = reinterpret_cast<unsigned short *>(ctx->d_B);
         unsigned short *Xl = Xh + (size_t)rows_pad * T;
         unsigned *sgn = reinterpret_cast<unsigned *>(Xl + (size_t)rows_pad * T);
-        timer_begin(ctx, "basis", &ev);
-        rc = check_hip(ctx, launch_basis_split(ctx, ctx->d_y, T, np, K, ctx->d_w, Xh, Xl, sgn), "basis");
-        timer_end(ctx, "basis", ev);
-        if (rc) return rc;
-        timer_begin(ctx, "syrk", &ev);
-        const int terms = (wmode == 1 && ctx->cv_one_term) ? 1 : 3;
-        rc = check_hip(ctx, launch_syrk_x3(ctx, ctx->nbasis, Xh, Xl, sgn, K, ctx->d_H, ctx->d_Hc, terms),
-                       "syrk");
-        timer_end(ctx, "syrk", ev);
-        if (rc) return rc;
+        for (int pass = 0; pass < (matched ? 2 : 1); ++pass) {
+            if (pass == 1) {
+                // then -w_nn, scaled, with the surrogate term's one product
+                timer_begin(ctx, "mlp", &ev);
+                rc = check_hip(ctx, launch_mlp_weights(ctx, ctx->d_y, T, np, nullptr, 0, -scale, ctx->d_w),
+                               "mlp");
+                timer_end(ctx, "mlp", ev);
+                if (rc) return rc;
+            }
+            timer_begin(ctx, "basis", &ev);
+            rc = check_hip(ctx, launch_basis_split(ctx, ctx->d_y, T, np, K, ctx->d_w, Xh, Xl, sgn), "basis");
+            timer_end(ctx, "basis", ev);
+            if (rc) return rc;
+            timer_begin(ctx, "syrk", &ev);
+            const int terms = ((wmode == 1 && ctx->cv_one_term) || pass == 1) ? 1 : 3;
+            rc = check_hip(ctx, launch_syrk_x3(ctx, ctx->nbasis, Xh, Xl, sgn, K, ctx->d_H, ctx->d_Hc, terms),
+                           "syrk");
+            timer_end(ctx, "syrk", ev);
+            if (rc) return rc;
+        }
     } else {
         timer_begin(ctx, "basis", &ev);
         rc = check_hip(ctx, launch_basis(ctx, ctx->d_y, T, np, ctx->d_B, T, fp32), "basis");

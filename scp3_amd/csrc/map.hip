@@ -72,10 +72,11 @@ hipError_t launch_map(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts
     return hipGetLastError();
 }
 
-// bV = E/autokcalpmol - 0.5 * sum_k y_k (omega_k y_k); SCP3.cpp:219-223.
+// bV = E/autokcalpmol - 0.5 * sum_k y_k (omega_k y_k); SCP3.cpp:219-223;
+// times `scale` (1 leaves bV bit for bit).
 __global__ void __launch_bounds__(256)
     k_weight(const double *__restrict__ E, const double *__restrict__ y, int64_t ldy,
-             const double *__restrict__ omega, int nmodes0, int64_t npts, int64_t npad,
+             const double *__restrict__ omega, int nmodes0, int64_t npts, int64_t npad, double scale,
              double *__restrict__ w)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -89,16 +90,17 @@ __global__ void __launch_bounds__(256)
         const double yk = y[(size_t)k * ldy + i];
         h += yk * (omega[k] * yk);
     }
-    w[i] = E[i] / kAutokcalpmol - 0.5 * h;
+    const double bV = E[i] / kAutokcalpmol - 0.5 * h;
+    w[i] = scale * bV;
 }
 
 hipError_t launch_weight(scp3g_ctx *ctx, const double *E, const double *y, int64_t ldy,
-                         int64_t npts, double *w)
+                         int64_t npts, double *w, double scale)
 {
     const int64_t npad = (npts + 255) / 256 * 256;
     if (npad <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_weight, dim3((unsigned)(npad / 256)), dim3(256), 0, ctx->stream, E, y,
-                       ldy, ctx->d_omega, ctx->nmodes0, npts, npad, w);
+                       ldy, ctx->d_omega, ctx->nmodes0, npts, npad, scale, w);
     return hipGetLastError();
 }
 

@@ -411,7 +411,7 @@ __global__ void k_to_bf16(const float *__restrict__ W, int rows, int cols,
 // mode 0: the surrogate weight w_nn = V0 + t_mean + t_std * f; mode 1: the
 // correction weight w - w_nn = V - V_nn from the energies E (kcal/mol,
 // SCP3.cpp:222-223); mode 2: the surrogate potential V_nn = w_nn + 1/2
-// sum_k omega_k y_k^2 (hartree).
+// sum_k omega_k y_k^2 (hartree).  Every mode multiplies by `scale`.
 __global__ void k_mlp_weights(const float *__restrict__ f, const double *__restrict__ y,
                               int64_t ldy, const double *__restrict__ omega, int M, double base,
                               double tstd, int64_t npts, int64_t npad, const double *__restrict__ E,

@@ -171,7 +171,7 @@ hipError_t launch_map(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts
 hipError_t launch_pes(scp3g_ctx *ctx, int pes, int nw, int64_t npts, const void *crd,
                       int64_t ldc, int fp32, double *E);
 hipError_t launch_weight(scp3g_ctx *ctx, const double *E, const double *y, int64_t ldy,
-                         int64_t npts, double *w);
+                         int64_t npts, double *w, double scale = 1.0);
 hipError_t launch_basis(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts, void *B,
                         int64_t ldb, int fp32);
 hipError_t launch_syrk(scp3g_ctx *ctx, int mode, int64_t n, const void *B, int64_t ldb,

@@ -79,7 +79,12 @@ def test_time_to_half_wavenumber(record):
     out = bench.time_to_half_wavenumber(dict(PR), res, 10)
     assert out["fp64"]["s"] == pytest.approx(32 / 10.0 + 0.5)
     assert out["mixed"]["s"] == pytest.approx(32 / 100.0 + 0.5)
-    # surrogate rate: 2^cv_nn points in the step's time less 2^cv_t true points at the mixed rate
-    rate_nn = 16 / (2.0 - 8 / 100.0)
-    assert out["cv"]["s"] == pytest.approx(3.0 + 16 / rate_nn + 8 / 100.0 + 0.5)
+    # one-product surrogate (the default): the correction's 2^cv_t true
+    # points are each a mixed-form point and a surrogate point
+    rate_nn = (16 + 8) / (2.0 - 8 / 100.0)
+    assert out["cv"]["s"] == pytest.approx(3.0 + (16 + 8) / rate_nn + 8 / 100.0 + 0.5)
     assert out["speedup_cv_vs_fp64"] == pytest.approx(out["fp64"]["s"] / out["cv"]["s"])
+    # three-product surrogate: the correction is one mixed-form pass
+    out3 = bench.time_to_half_wavenumber(dict(PR, cv_terms=3), res, 10)
+    rate3 = 16 / (2.0 - 8 / 100.0)
+    assert out3["cv"]["s"] == pytest.approx(3.0 + 16 / rate3 + 8 / 100.0 + 0.5)

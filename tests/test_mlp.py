@@ -166,6 +166,55 @@ def test_control_variate_identity(gpu_ctx_factory, hexamer):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("P", [1 << 14, 3 * 4096 + 100])
+def test_control_variate_one_product_identity(gpu_ctx_factory, hexamer, P):
+    """Mixed form with the surrogate term on one BF16 product: the
+    correction term subtracts w_nn with that same product, so with the same
+    points in both terms sum 1p(w_nn) + sum [3p(bV) - 1p(w_nn)] = sum 3p(bV)
+    and the control variate equals the plain mixed estimator.  At P = 4^7
+    the weights' scale 1/P is a power of 4 and the split-BF16 operands scale
+    exactly: the two agree to 1e-8 of max|H| (5e-10 measured); at other P
+    to the three-product form's precision.  A correction taking w_nn
+    with three products would leave the one-product rounding of the
+    surrogate term, measured here as the surrogate term with one against
+    three products."""
+    import scp3_amd
+    print("one-product control variate case: P=%d" % P, flush=True)
+    ctx = gpu_ctx_factory(0)
+    _small_basis(ctx, hexamer, scp3_amd.MIXED)
+    ctx.set_tile(0)
+    ctx.mlp_init(64, seed=3)
+    ctx.mlp_sample_data(1 << 30, 4096)
+    ctx.mlp_train(50, 1024, 1e-3)
+    skip = 1 << 30
+    ctx.set_cv_terms(1)
+    ctx.reset()
+    ctx.accumulate(skip, P)
+    H = ctx.h_download()
+    ctx.reset()
+    ctx.accumulate_cv(skip, P, skip, P)
+    Hcv = ctx.h_download()
+    ctx.reset()
+    ctx.accumulate_surrogate(skip, P)
+    S1 = ctx.h_download()
+    ctx.set_cv_terms(3)
+    ctx.reset()
+    ctx.accumulate_surrogate(skip, P)
+    S3 = ctx.h_download()
+    ctx.set_cv_terms(1)
+    err = np.abs(Hcv - H).max()
+    model = np.abs(S1 - S3).max()
+    print("one-product control variate: |H_cv - H| %.3e, one-product rounding %.3e, max|H| %.3e"
+          % (err, model, np.abs(H).max()))
+    if P == 1 << 14:
+        assert err < 1e-8 * np.abs(H).max()
+        assert err < 1e-3 * model
+    else:
+        assert err < 2e-5 * np.abs(H).max()
+        assert err < 0.1 * model
+
+
+@pytest.mark.gpu
 def test_control_variate_beats_plain_qmc(gpu_ctx_factory, hexamer):
     """At the same number of true-potential points, the trained surrogate as
     control variate gives a smaller error of H and of the levels than plain
