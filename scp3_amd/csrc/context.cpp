@@ -685,7 +685,7 @@ int scp3g::run_tile(scp3g_ctx *ctx, int64_t np, int wmode, double scale)
             return fail(ctx, -4, "the gradient estimator has no surrogate or energy-input form "
                                  "(scp3g_accumulate_gradients takes energies with gradients)");
         if (ctx->precision == SCP3G_MIXED && !ctx->mixed_fast)
-            return fail(ctx, -4, "the gradient estimator's mixed form needs the LDS-tiled SYRK (mixed kernel 1 or 2)");
+            return fail(ctx, -4, "the gradient estimator's mixed form needs the LDS-tiled SYRK (mixed kernel 1)");
         if (ctx->nmodes3 > ctx->nmodes2) return fail(ctx, -4, "the gradient estimator needs nmodes3 <= nmodes2");
         return run_tile_grad(ctx, np, wmode == 4);
     }
