@@ -741,12 +741,14 @@ basis:
                 timer_end(ctx, "mlp", ev);
                 if (rc) return rc;
             }
+            // a one-product SYRK reads the hi plane only
+            const int terms = ((wmode == 1 && ctx->cv_one_term) || pass == 1) ? 1 : 3;
             timer_begin(ctx, "basis", &ev);
-            rc = check_hip(ctx, launch_basis_split(ctx, ctx->d_y, T, np, K, ctx->d_w, Xh, Xl, sgn), "basis");
+            rc = check_hip(ctx, launch_basis_split(ctx, ctx->d_y, T, np, K, ctx->d_w, Xh, terms == 1 ? nullptr : Xl, sgn),
+                           "basis");
             timer_end(ctx, "basis", ev);
             if (rc) return rc;
             timer_begin(ctx, "syrk", &ev);
-            const int terms = ((wmode == 1 && ctx->cv_one_term) || pass == 1) ? 1 : 3;
             rc = check_hip(ctx, launch_syrk_x3(ctx, ctx->nbasis, Xh, Xl, sgn, K, ctx->d_H, ctx->d_Hc, terms),
                            "syrk");
             timer_end(ctx, "syrk", ev);

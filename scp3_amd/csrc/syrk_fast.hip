@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // x = sqrt|w| * bra in FP32, split into hi/lo BF16 planes in the k-blocked
-// layout.  A block holds 64 points (two 32-point chunk blocks) and a range of
+// layout (Xl null: the hi plane only, for a one-product SYRK).  A block holds 64 points (two 32-point chunk blocks) and a range of
 // rows.  Phase 1 puts the Hermite values h1..h3 of the selected modes and
 // sqrt|w| of its points in LDS.  Phase 2: thread t takes chunk block t / 128,
 // points 8 (t % 4) .. +7 of it, and row (t % 128) / 4 of every 32-row pass;
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(256)
     int64_t a1 = a0 + rows_per_block;
     if (a1 > rows_pad) a1 = rows_pad;
     uint4 *ph = Xh + (size_t)kb * rows_pad * 4 + pg;
-    uint4 *pl = Xl + (size_t)kb * rows_pad * 4 + pg;
+    uint4 *pl = Xl ? Xl + (size_t)kb * rows_pad * 4 + pg : nullptr;
     for (int64_t a = a0 + rl; a < a1; a += 32) {
         float v[8];
         if (a < n) {
@@ -152,8 +152,9 @@ __global__ void __launch_bounds__(256)
         }
         ph[a * 4] = make_uint4(hh[0] | (hh[1] << 16), hh[2] | (hh[3] << 16), hh[4] | (hh[5] << 16),
                                hh[6] | (hh[7] << 16));
-        pl[a * 4] = make_uint4(ll[0] | (ll[1] << 16), ll[2] | (ll[3] << 16), ll[4] | (ll[5] << 16),
-                               ll[6] | (ll[7] << 16));
+        if (Xl)
+            pl[a * 4] = make_uint4(ll[0] | (ll[1] << 16), ll[2] | (ll[3] << 16), ll[4] | (ll[5] << 16),
+                                   ll[6] | (ll[7] << 16));
     }
 }
 
