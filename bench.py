@@ -109,6 +109,7 @@ def parse():
     ap.add_argument("--cv-t", type=int, default=None, help="log2 of true-potential points of the control variate")
     ap.add_argument("--cv-nn", type=int, default=None, help="log2 of surrogate points of the control variate")
     ap.add_argument("--mlp-steps", type=int, default=None, help="Adam steps of the surrogate's training")
+    ap.add_argument("--mlp-pairs", type=int, default=None, help="log2 of the surrogate's training pairs")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     return ap.parse_args()
@@ -591,6 +592,8 @@ def main():
         pr["cv_nn"] = a.cv_nn
     if a.mlp_steps is not None:
         pr["mlp_steps"] = a.mlp_steps
+    if a.mlp_pairs is not None:
+        pr["mlp_pairs"] = a.mlp_pairs
     pr["cv_terms"] = a.cv_terms
     if a.converge_summary:
         # states: ground, singles of the selected modes, doubles of the first
