@@ -423,7 +423,8 @@ __device__ __forceinline__ void kahan_add2(double &h, double &c, double v)
 // the whole K range and Kahan-adds its tile into H.
 //
 // TERMS = 3: hi.hi + hi.lo + lo.hi; TERMS = 1: hi.hi only (relative error
-// ~2^-8 per product; used for the surrogate term of the control variate)
+// ~2^-8 per product; used for the surrogate term of the control variate and
+// for w_nn in its correction term, where the two roundings cancel)
 template <int TERMS>
 __global__ void __launch_bounds__(512, 1)
     k_syrk_x3(const uint4 *__restrict__ Xh, const uint4 *__restrict__ Xl,
