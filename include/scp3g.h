@@ -255,6 +255,17 @@ int scp3g_accumulate_surrogate(scp3g_ctx *ctx, int64_t skip, int64_t npts);
  * scp3g_accumulate_surrogate in another context it lets a caller grow the
  * two terms separately; no reference counterpart */
 int scp3g_accumulate_correction(scp3g_ctx *ctx, int64_t skip, int64_t npts);
+/* the running sum, normalised (sum / normaliser, no HO diagonal), is added
+ * to an earlier fixed term (none at first), the total becomes the fixed
+ * term of every later Mout (scp3g_h_download with raw = 0) and level solve
+ * (scp3g_lowest_levels, _sub), and the running sum restarts at zero: after
+ * scp3g_accumulate_correction, a surrogate term grown by
+ * scp3g_accumulate_surrogate then gives the control variate's H = S / P_nn
+ * + C / P_t at every checkpoint.  Energy estimator only; a basis, system or
+ * estimator change drops the term; no reference counterpart */
+int scp3g_hold_as_offset(scp3g_ctx *ctx);
+/* drop the fixed term of scp3g_hold_as_offset */
+int scp3g_clear_offset(scp3g_ctx *ctx);
 
 /* ---- per-stage entry points (device pointers; used by parity tests) ---- */
 

@@ -59,6 +59,8 @@ _SIGS = {
                                           ctypes.c_int64]),
     "scp3g_accumulate_surrogate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "scp3g_accumulate_correction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
+    "scp3g_hold_as_offset": (ctypes.c_int, [ctypes.c_void_p]),
+    "scp3g_clear_offset": (ctypes.c_int, [ctypes.c_void_p]),
     "scp3g_points_coords": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
     "scp3g_accumulate_energies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
     "scp3g_accumulate_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
@@ -314,6 +316,14 @@ class Context:
     def accumulate_correction(self, skip, npts):
         """correction term alone: sum B B^T (bV - w_nn), normaliser += npts"""
         self.check(self.L.scp3g_accumulate_correction(self._h, int(skip), int(npts)))
+
+    def hold_as_offset(self):
+        """the running sum, normalised, becomes a fixed term of every later
+        Mout and level solve; the running sum restarts at zero"""
+        self.check(self.L.scp3g_hold_as_offset(self._h))
+
+    def clear_offset(self):
+        self.check(self.L.scp3g_clear_offset(self._h))
 
     def accumulate_cv(self, skip_nn, P_nn, skip_t, P_t):
         self.check(self.L.scp3g_accumulate_cv(self._h, int(skip_nn), int(P_nn), int(skip_t), int(P_t)))

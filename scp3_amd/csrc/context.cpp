@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -184,6 +185,7 @@ static void free_sums(scp3g_ctx *ctx)
     dfree(ctx->d_Heff);
     dfree(ctx->d_mout);
     dfree(ctx->d_diag);
+    dfree(ctx->d_offset);
     ctx->h_n = 0;
     ctx->points_done = 0;
     ctx->norm_sum = 0.0;
@@ -542,6 +544,38 @@ int scp3g_set_tile(scp3g_ctx *ctx, int tile)
     if (tile < 0 || tile % 256) return fail(ctx, -4, "tile must be 0 or a positive multiple of 256");
     ctx->tile = tile;
     free_work(ctx);
+    return 0;
+}
+
+// The running sum, normalised, becomes a fixed term of every later Mout and
+// level solve, and the running sum restarts at zero (a control variate's
+// correction term C / P_t under a surrogate term grown checkpoint by
+// checkpoint).  Energy estimator only.
+int scp3g_hold_as_offset(scp3g_ctx *ctx)
+{
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (ctx->estimator != SCP3G_EST_ENERGY) return fail(ctx, -4, "a fixed term needs the energy (-G) estimator");
+    if (!ctx->d_H || ctx->points_done <= 0 || !(ctx->norm_sum > 0.0)) return fail(ctx, -6, "no points accumulated");
+    hipSetDevice(ctx->device);
+    const int64_t n = ctx->nbasis;
+    const size_t bytes = sizeof(double) * (size_t)n * (size_t)n;
+    // this sum / its normaliser + the earlier fixed term (if any), upper
+    // triangle, into the Mout staging buffer, which then becomes the fixed
+    // term (and the earlier one the staging buffer)
+    if (!ctx->d_mout && (rc = check_hip(ctx, hipMalloc(&ctx->d_mout, bytes), "alloc Mout"))) return rc;
+    if ((rc = check_hip(ctx, launch_finalize(ctx, ctx->d_H, n, ctx->norm_sum, nullptr, ctx->d_mout, ctx->d_offset),
+                        "offset")))
+        return rc;
+    std::swap(ctx->d_offset, ctx->d_mout);
+    return scp3g_reset(ctx);
+}
+
+int scp3g_clear_offset(scp3g_ctx *ctx)
+{
+    if (!ctx) return -1;
+    hipSetDevice(ctx->device);
+    dfree(ctx->d_offset);
     return 0;
 }
 
@@ -955,7 +989,8 @@ int scp3g_h_download(scp3g_ctx *ctx, double *M_host, int raw)
         return rc;
     hipEvent_t ev;
     timer_begin(ctx, "finalize", &ev);
-    rc = check_hip(ctx, launch_finalize(ctx, Hs, n, ctx->norm_sum, ctx->d_diag, ctx->d_mout), "finalize");
+    rc = check_hip(ctx, launch_finalize(ctx, Hs, n, ctx->norm_sum, ctx->d_diag, ctx->d_mout, ctx->d_offset),
+                   "finalize");
     timer_end(ctx, "finalize", ev);
     // dg is read by the queued upload: wait before returning either way
     const int rs = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");

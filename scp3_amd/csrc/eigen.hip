@@ -30,12 +30,13 @@ constexpr int kMaxSplit = 16; // split-K limit of A X
 
 int64_t lda_of(int64_t n) { return (n + kLdA - 1) / kLdA * kLdA; }
 
-// A(r, c) = Mout(min, max) / P (+ diag on r == c) from the upper-triangle sum
-// H, through 32 x 32 LDS tiles so that both the read of the mirrored
-// triangle and the write stay coalesced.  A has leading dimension lda
-// (a multiple of 32); its padding rows n..lda-1 are set to zero.
+// A(r, c) = Mout(min, max) / P (+ O(min, max), + diag on r == c) from the
+// upper-triangle sum H and the optional fixed term O (same layout), through
+// 32 x 32 LDS tiles so that both the read of the mirrored triangle and the
+// write stay coalesced.  A has leading dimension lda (a multiple of 32); its
+// padding rows n..lda-1 are set to zero.
 __global__ void __launch_bounds__(256)
-    k_sym_full(const double *__restrict__ H, int64_t ldh, int64_t n, double P,
+    k_sym_full(const double *__restrict__ H, const double *__restrict__ O, int64_t ldh, int64_t n, double P,
                const double *__restrict__ diag, double *__restrict__ A, int64_t lda)
 {
     __shared__ double t[32][33];
@@ -46,8 +47,9 @@ __global__ void __launch_bounds__(256)
         for (int k = ty; k < 32; k += 8) {
             const int64_t r = br + tx, c = bc + k;
             if (r < n && c < n) {
-                double v = (r <= c) ? H[(size_t)c * ldh + r] / P : 0.0;
-                if (r > c) v = H[(size_t)r * ldh + c] / P;
+                const size_t e = r <= c ? (size_t)c * ldh + r : (size_t)r * ldh + c;
+                double v = H[e] / P;
+                if (O) v += O[e];
                 if (r == c) v += diag[r];
                 A[(size_t)c * lda + r] = v;
             } else if (r < lda && c < n) {
@@ -59,7 +61,7 @@ __global__ void __launch_bounds__(256)
         // (rows bc.., cols br..) coalesced, transpose through LDS
         for (int k = ty; k < 32; k += 8) {
             const int64_t r = bc + tx, c = br + k;  // element H(r, c), r < c
-            t[k][tx] = (r < n && c < n) ? H[(size_t)c * ldh + r] / P : 0.0;
+            t[k][tx] = (r < n && c < n) ? H[(size_t)c * ldh + r] / P + (O ? O[(size_t)c * ldh + r] : 0.0) : 0.0;
         }
         __syncthreads();
         for (int k = ty; k < 32; k += 8) {
@@ -553,8 +555,8 @@ struct Dav {
 // tested).  The correction block is orthonormalised on the GPU (CholQR2 with
 // column dropping); a dropped column stays in V as a zero column, which the
 // host leaves out of the projected problem.
-static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t n, double P,
-                        const std::vector<double> &dg, int k, double tol, double *evals,
+static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, const double *Osrc, int64_t ldh, int64_t n,
+                        double P, const std::vector<double> &dg, int k, double tol, double *evals,
                         double *evecs)
 {
     if (k < 1 || k > 28 || k > n) return fail(ctx, -4, "k must be in 1..min(28, n)");
@@ -632,7 +634,7 @@ static int davidson_run(scp3g_ctx *ctx, const double *Hsrc, int64_t ldh, int64_t
         hipEvent_t ev;
         timer_begin(ctx, "symfill", &ev);
         dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
-        hipLaunchKernelGGL(k_sym_full, grid, dim3(256), 0, ctx->stream, Hsrc, ldh, n, P, dv.diag,
+        hipLaunchKernelGGL(k_sym_full, grid, dim3(256), 0, ctx->stream, Hsrc, Osrc, ldh, n, P, dv.diag,
                            d_A, lda);
         timer_end(ctx, "symfill", ev);
         DV_CHECK(hipGetLastError(), "symfill");
@@ -843,7 +845,7 @@ extern "C" int scp3g_lowest_levels(scp3g_ctx *ctx, int k, double tol, double *ev
     const double *Hs = nullptr;
     int rc = read_sums(ctx, &Hs);
     if (rc) return rc;
-    return davidson_run(ctx, Hs, ctx->nbasis, ctx->nbasis, ctx->norm_sum,
+    return davidson_run(ctx, Hs, ctx->d_offset, ctx->nbasis, ctx->nbasis, ctx->norm_sum,
                         ho_diagonal(ctx), k, tol, evals, evecs);
 }
 
@@ -859,7 +861,7 @@ extern "C" int scp3g_lowest_levels_sub(scp3g_ctx *ctx, int64_t nsub, int k, doub
     const double *Hs = nullptr;
     int rc = read_sums(ctx, &Hs);
     if (rc) return rc;
-    return davidson_run(ctx, Hs, ctx->nbasis, nsub, ctx->norm_sum, dg, k, tol,
+    return davidson_run(ctx, Hs, ctx->d_offset, ctx->nbasis, nsub, ctx->norm_sum, dg, k, tol,
                         evals, nullptr);
 }
 
@@ -880,7 +882,7 @@ extern "C" int scp3g_matrix_lowest_levels(scp3g_ctx *ctx, int64_t n, const doubl
     int rc = check_hip(ctx, hipMalloc(&d_M, U.size() * 8), "alloc M");
     if (rc) return rc;
     rc = check_hip(ctx, hipMemcpy(d_M, U.data(), U.size() * 8, hipMemcpyHostToDevice), "upload M");
-    if (!rc) rc = davidson_run(ctx, d_M, n, n, 1.0, std::vector<double>((size_t)n, 0.0), k, tol, evals, evecs);
+    if (!rc) rc = davidson_run(ctx, d_M, nullptr, n, n, 1.0, std::vector<double>((size_t)n, 0.0), k, tol, evals, evecs);
     hipFree(d_M);
     return rc;
 }

@@ -20,7 +20,13 @@
 // the PES's gradient overload without -G), --stein (the gradient estimator with every element from the Stein
 // identity instead of the reference's placement of five element families),
 // --scramble SEED (affine digit scrambling of the Sobol points, in place of
-// the reference's external scrambled -r files).
+// the reference's external scrambled -r files), --cv-true p (with -G: the
+// learned control variate; an MLP surrogate of the potential is trained on
+// the sampler's own (q, V) pairs, the correction term runs over 2^p
+// true-potential points first, and the -N points are surrogate points,
+// H = S / P + C / 2^p at every checkpoint; --cv-pairs p, 2^p training pairs,
+// default 18; --cv-steps s, Adam steps, default 20000; --cv-hidden h, width
+// of the three hidden layers, default 256).
 // Files: omega0.dat, E0_sd.dat (every 2^14 points), E0_t.dat (empty),
 // sM_%07d.h5 (every 2^17 points), levels.dat (this driver's final levels).
 #include <getopt.h>
@@ -48,6 +54,7 @@ struct Opts {
     bool scrambled = false;
     unsigned long long scramble = 0;
     int precision = SCP3G_FP64;
+    int cv_true = -1, cv_pairs = 18, cv_steps = 20000, cv_hidden = 256;
 };
 
 // parse_mode_description (SCP3.cpp:63-89): "a-b,c" -> a..b, c
@@ -92,6 +99,10 @@ Opts parse(int argc, char **argv)
                           {"cpu-pes", no_argument, nullptr, 1003},
                           {"stein", no_argument, nullptr, 1004},
                           {"scramble", required_argument, nullptr, 1005},
+                          {"cv-true", required_argument, nullptr, 1006},
+                          {"cv-pairs", required_argument, nullptr, 1007},
+                          {"cv-steps", required_argument, nullptr, 1008},
+                          {"cv-hidden", required_argument, nullptr, 1009},
                           {nullptr, 0, nullptr, 0}};
     Opts o;
     int ch;
@@ -132,6 +143,16 @@ Opts parse(int argc, char **argv)
             o.scrambled = true;
             o.scramble = strtoull(optarg, nullptr, 0);
             break;
+        case 1006:
+            o.cv_true = atoi(optarg);
+            if (o.cv_true < 0 || o.cv_true > 40) die("--cv-true <p>: 2^p true-potential points, p in 0..40");
+            break;
+        case 1007:
+            o.cv_pairs = atoi(optarg);
+            if (o.cv_pairs < 0 || o.cv_pairs > 30) die("--cv-pairs <p>: 2^p training pairs, p in 0..30");
+            break;
+        case 1008: o.cv_steps = atoi(optarg); break;
+        case 1009: o.cv_hidden = atoi(optarg); break;
         default: die("Unknown option: " + std::to_string(ch));
         }
     }
@@ -161,6 +182,8 @@ int main(int argc, char **argv)
     } else if (o.pes != "qtip4pf" && o.pes != "ttm3f") {
         die("Unknown H2O potential: " + o.pes);
     }
+    if (o.cv_true >= 0 && (o.gradient || o.cpu_pes || !o.rng_file.empty()))
+        die("--cv-true needs -G, the device potential and Sobol points (no --cpu-pes, no -r)");
 
     cli::Input in;
     std::string err;
@@ -195,13 +218,27 @@ int main(int argc, char **argv)
         check(ctx, scp3g_set_estimator(ctx, o.stein ? SCP3G_EST_GRADIENT_STEIN : SCP3G_EST_GRADIENT),
               "set_estimator");
     check(ctx, scp3g_reset(ctx), "reset");
+    if (o.cv_true >= 0) {
+        // the surrogate, trained on pairs from far along the sequence; the
+        // correction term over the first 2^cv_true points, held as a fixed
+        // term; then the surrogate term over the -N points (the one BF16
+        // product in the mixed form, matched in the correction)
+        check(ctx, scp3g_mlp_init(ctx, o.cv_hidden, 0), "mlp init");
+        check(ctx, scp3g_mlp_sample_data(ctx, (int64_t)1 << 40, (int64_t)1 << o.cv_pairs), "mlp training set");
+        check(ctx, scp3g_mlp_train(ctx, o.cv_steps, 4096, 1e-3, nullptr), "mlp training");
+        if (o.precision == SCP3G_MIXED) check(ctx, scp3g_set_cv_terms(ctx, 1), "cv terms");
+        check(ctx, scp3g_accumulate_correction(ctx, o.skip, (int64_t)1 << o.cv_true), "correction term");
+        check(ctx, scp3g_hold_as_offset(ctx), "hold correction term");
+    }
     int64_t n = 0, n1 = 0, n2s = 0;
     scp3g_basis_size(ctx, &n);
     scp3g_sub_basis_size(ctx, 1, &n1);
     scp3g_sub_basis_size(ctx, 2, &n2s);
     std::cerr << "SCP3 (GPU, " << (o.gradient ? (o.stein ? "gradient, Stein" : "gradient") : "-G") << "): " << N / 3 << " waters, " << modes.size() << " modes, " << n
               << " states, " << o.nsobol << " points, " << (o.precision ? "mixed" : "fp64")
-              << " form, " << o.pes << "\n";
+              << " form, " << o.pes;
+    if (o.cv_true >= 0) std::cerr << ", control variate (2^" << o.cv_true << " true-potential points)";
+    std::cerr << "\n";
 
     std::ofstream e0sd("E0_sd.dat"), e0t("E0_t.dat");
     e0sd << std::fixed << std::setprecision(10);
@@ -257,6 +294,8 @@ int main(int argc, char **argv)
             for (double &v : nr) rng >> v;
             if (!rng) die("rng-file ended early");
             check(ctx, scp3g_accumulate_normals(ctx, nr.data(), span), "accumulate");
+        } else if (o.cv_true >= 0) {
+            check(ctx, scp3g_accumulate_surrogate(ctx, o.skip + i, span), "accumulate");
         } else {
             check(ctx, scp3g_accumulate(ctx, o.skip + i, span), "accumulate");
         }

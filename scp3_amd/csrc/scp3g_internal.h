@@ -114,6 +114,10 @@ struct scp3g_ctx {
     // Mout staging of scp3g_h_download (n x n) and the diagonal it adds,
     // kept with the running sums
     double *d_mout = nullptr, *d_diag = nullptr;
+    // fixed term added to every finalised Mout (scp3g_hold_as_offset): an
+    // earlier normalised sum, upper triangle, leading dimension n; freed
+    // with the running sums
+    double *d_offset = nullptr;
     int64_t points_done = 0;
     // normaliser of the running sum: the number of points for the plain
     // estimator, the sum of per-point scales for the control-variate one
@@ -177,7 +181,7 @@ hipError_t launch_basis(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t np
 hipError_t launch_syrk(scp3g_ctx *ctx, int mode, int64_t n, const void *B, int64_t ldb,
                        const double *w, int64_t npts, double *H, double *Hc);
 hipError_t launch_finalize(scp3g_ctx *ctx, const double *H, int64_t n, double scale,
-                           const double *diag_add, double *out);
+                           const double *diag_add, double *out, const double *offset = nullptr);
 
 hipError_t launch_mlp_weights(scp3g_ctx *ctx, const double *y, int64_t ldy, int64_t npts,
                               const double *E, int mode, double scale, double *w);

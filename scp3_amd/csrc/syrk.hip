@@ -107,7 +107,8 @@ __global__ void __launch_bounds__(256)
 // Mout = H / P (+ diag) on the upper triangle, zero below (SCP3.cpp:237-239).
 __global__ void __launch_bounds__(256)
     k_finalize(const double *__restrict__ H, int64_t n, double P,
-               const double *__restrict__ diag_add, double *__restrict__ out)
+               const double *__restrict__ diag_add, const double *__restrict__ offset,
+               double *__restrict__ out)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * n) return;
@@ -115,6 +116,7 @@ __global__ void __launch_bounds__(256)
     double v = 0.0;
     if (r <= c) {
         v = H[idx] / P;
+        if (offset) v += offset[idx];
         if (r == c && diag_add) v += diag_add[r];
     }
     out[idx] = v;
@@ -218,12 +220,12 @@ hipError_t launch_syrk(scp3g_ctx *ctx, int mode, int64_t n, const void *B, int64
 }
 
 hipError_t launch_finalize(scp3g_ctx *ctx, const double *H, int64_t n, double scale,
-                           const double *diag_add, double *out)
+                           const double *diag_add, double *out, const double *offset)
 {
     const int64_t tot = n * n;
     if (tot <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_finalize, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                       ctx->stream, H, n, scale, diag_add, out);
+                       ctx->stream, H, n, scale, diag_add, offset, out);
     return hipGetLastError();
 }
 

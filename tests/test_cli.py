@@ -179,3 +179,40 @@ def test_scp3_mode_list_skip_block_and_triples(tmp_path, modes):
         c.accumulate(1 << 20, 17000)
         ev = c.lowest_levels(5)
     np.testing.assert_allclose(lv[:, 0], ev, rtol=0, atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_scp3_control_variate_run(tmp_path, precision):
+    """SCP3 -G --cv-true: the surrogate trained on the sampler's pairs, the
+    correction term over 2^12 true-potential points held, the surrogate term
+    over the -N points; levels.dat against scp3g_accumulate_cv through the
+    binding with the same network, training and seeds."""
+    import scp3_amd
+    from scp3_amd import system
+    print("cli case: dimer control variate -N 14 --cv-true 12", precision, flush=True)
+    r = run(["SCP3", "-G", "-N", "14", "-2", "12", "--levels", "5", "--precision", precision, "--cv-true", "12",
+             "--cv-pairs", "12", "--cv-steps", "100", DIMER], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert "control variate" in r.stderr
+    lv = np.loadtxt(tmp_path / "levels.dat")
+    e0 = np.atleast_2d(np.loadtxt(tmp_path / "E0_sd.dat"))   # one checkpoint
+    sysd = system.prepare(DIMER, nmodes2=12, nmodes3=0)
+    with scp3_amd.Context(0) as c:
+        c.set_system(sysd["x0"], sysd["omega"], sysd["MUa"])
+        c.set_basis(sysd["modes"], 12, 0)
+        c.set_pes("qtip4pf")
+        c.set_precision(scp3_amd.MIXED if precision == "mixed" else scp3_amd.FP64)
+        c.mlp_init(256, seed=0)
+        c.mlp_sample_data(1 << 40, 1 << 12)
+        c.mlp_train(100, 4096, 1e-3)
+        c.set_cv_terms(1)
+        c.reset()
+        c.accumulate_cv(1 << 30, 1 << 14, 1 << 30, 1 << 12)
+        ev = c.lowest_levels(5)
+        e1 = c.lowest_levels_sub(c.sub_basis_size(1), 1)[0]
+    # FP64: the same sums up to rounding; mixed: the binding's single sum
+    # scales the weights by 1/P (powers of 4 here) before the BF16 split
+    tol = 1e-10 if precision == "fp64" else 1e-8
+    np.testing.assert_allclose(lv[:, 0], ev, rtol=0, atol=tol)
+    assert abs(e0[-1, 1] - e1 * scp3_amd.AUTOCM) < 1e-6 + tol * scp3_amd.AUTOCM

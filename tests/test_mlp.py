@@ -215,6 +215,51 @@ def test_control_variate_one_product_identity(gpu_ctx_factory, hexamer, P):
 
 
 @pytest.mark.gpu
+def test_hold_as_offset_gives_control_variate(gpu_ctx_factory, hexamer):
+    """The correction term held as a fixed term under a surrogate term grown
+    in the running sum: Mout, the levels and the sub-block levels equal the
+    control variate's S / P_nn + C / P_t (scp3g_accumulate_cv on the same
+    seeds, FP64); a second hold adds to the first; raw downloads and a
+    cleared term give the running sum alone."""
+    import scp3_amd
+    ctx = gpu_ctx_factory(0)
+    _small_basis(ctx, hexamer, scp3_amd.FP64)
+    ctx.mlp_init(64, seed=5)
+    ctx.mlp_sample_data(1 << 40, 4096)
+    ctx.mlp_train(50, 1024, 1e-3)
+    skip_nn, Pnn, skip_t, Pt = 1 << 30, 2 * 4096 + 300, (1 << 30) + 77, 4096 + 11
+    ctx.reset()
+    ctx.accumulate_cv(skip_nn, Pnn, skip_t, Pt)
+    Mref = ctx.h_download()
+    ev_ref = ctx.lowest_levels(5)
+    n1 = ctx.sub_basis_size(1)
+    e1_ref = ctx.lowest_levels_sub(n1, 1)[0]
+    ctx.reset()
+    with pytest.raises(scp3_amd.Scp3Error):
+        ctx.hold_as_offset()                      # nothing accumulated
+    ctx.accumulate_correction(skip_t, Pt)
+    C = ctx.h_download(raw=True)
+    ctx.hold_as_offset()
+    assert ctx.points_done() == 0
+    ctx.accumulate_surrogate(skip_nn, Pnn)
+    S = np.triu(ctx.h_download(raw=True))         # the running sum alone
+    C = np.triu(C)
+    M = ctx.h_download()
+    scale = np.abs(Mref).max()
+    np.testing.assert_allclose(M, Mref, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(ctx.lowest_levels(5), ev_ref, rtol=0, atol=1e-10)
+    assert abs(ctx.lowest_levels_sub(n1, 1)[0] - e1_ref) < 1e-10
+    D = np.diag(np.diag(Mref - S / Pnn - C / Pt))  # the HO diagonal
+    # a second hold adds the running sum to the held term
+    ctx.hold_as_offset()
+    ctx.accumulate_correction(skip_t, Pt)
+    np.testing.assert_allclose(ctx.h_download(), Mref + C / Pt, rtol=0, atol=1e-12 * scale)
+    # cleared: the running sum alone
+    ctx.clear_offset()
+    np.testing.assert_allclose(ctx.h_download(), C / Pt + D, rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.gpu
 def test_control_variate_beats_plain_qmc(gpu_ctx_factory, hexamer):
     """At the same number of true-potential points, the trained surrogate as
     control variate gives a smaller error of H and of the levels than plain
