@@ -483,6 +483,8 @@ def write_convergence(pr, files, n, merge=False):
                      "levels_cm1": [round(float(v), 4) for v in ref],
                      "sem_cm1": [round(float(v), 4) for v in sem]}
     else:
+        if old is None:
+            raise SystemExit("--converge-summary: no stein_mixed runs for the reference levels and no record to merge")
         reference = old["reference"]
         ref = np.array(reference["levels_cm1"])
     rec = {"workload": "%s, %d states, scrambled Sobol points (skip 2^30)" % (pr["config"], n),

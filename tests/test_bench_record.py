@@ -88,3 +88,11 @@ def test_time_to_half_wavenumber(record):
     out3 = bench.time_to_half_wavenumber(dict(PR, cv_terms=3), res, 10)
     rate3 = 16 / (2.0 - 8 / 100.0)
     assert out3["cv"]["s"] == pytest.approx(3.0 + 16 / rate3 + 8 / 100.0 + 0.5)
+
+
+def test_summary_without_reference_fails_cleanly(tmp_path, monkeypatch):
+    monkeypatch.setattr(bench, "CONV_DIR", str(tmp_path))
+    f = tmp_path / "runs.jsonl"
+    f.write_text(lines([row("mixed", 0, 8, [0.1, 0, 0])]))
+    with pytest.raises(SystemExit):
+        bench.write_convergence(dict(PR), [str(f)], 10, merge=True)
