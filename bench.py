@@ -193,9 +193,11 @@ def converge(a, pr, sysd):
                 Ct = c.h_download(raw=True) / Pt
                 c.reset()
             for P in cps:
-                # 2^25-point pieces, each with a progress line on stderr
+                # pieces of 2^25 points (2^23 in the slower FP64 forms), each
+                # with a progress line on stderr
+                piece = 1 << (23 if mode == scp3_amd.FP64 else 25)
                 while done < P:
-                    m = min(1 << 25, P - done)
+                    m = min(piece, P - done)
                     c.sync()
                     t0 = time.perf_counter()
                     if form == "cv":
