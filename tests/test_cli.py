@@ -21,6 +21,15 @@ def test_gradient_estimator_unknown_cpu_potential(tmp_path):
     assert r.returncode != 0 and "Unknown H2O potential" in r.stderr
 
 
+@pytest.mark.parametrize("extra", [[], ["-G", "--cpu-pes"], ["-G", "--cv-true", "99"]])
+def test_control_variate_options_rejected(tmp_path, extra):
+    """--cv-true needs -G and the device potential, and a sane exponent;
+    the driver stops before it touches a GPU."""
+    args = ["SCP3", "-N", "10"] + (extra if "99" in extra else extra + ["--cv-true", "10"]) + [DIMER]
+    r = run(args, str(tmp_path))
+    assert r.returncode != 0 and "--cv-true" in r.stderr
+
+
 @pytest.mark.gpu
 def test_cpu_pes_gradient_run_matches_device_run(tmp_path):
     """--cpu-pes without -G: the gradient estimator from the host potential's
